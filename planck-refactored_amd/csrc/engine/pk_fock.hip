@@ -1,0 +1,603 @@
+// pk_fock.hip — host side of the MI355X Fock-build engine and its C ABI
+// (include/pk_fock.h).  Kernels live in eri_rys.h; this file builds the shell
+// pairs, the Schwarz table and the screened quartet work lists, and launches
+// the per-class kernels on the context's stream.
+//
+// Reference path replaced: src/integrals/base.h:57-93 (_compute_2e_fock[_uhf]),
+// src/integrals/os.cpp:1435-1625 and rys.cpp:701-824 (the Fock builds),
+// os.cpp:1703-1752 (Schwarz table), shellpair.cpp:7-22 (pairs).
+#include "pk_fock.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "eri_classes.h"
+
+// per-class launchers, defined in pk_kernels.hip
+#define PK_DECL(a, b, c, d, shard)                                                             \
+    void pk_launch_jk_##a##b##c##d(const pk::JKArgs &, int, hipStream_t, int &);               \
+    void pk_launch_eri_##a##b##c##d(const pk::EriArgs &, int, hipStream_t);
+PK_FOR_EACH_CLASS(PK_DECL)
+#undef PK_DECL
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define PK_HIP(expr)                                                                            \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            return fail(PK_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_));           \
+    } while (0)
+
+constexpr int kMaxL = PK_MAX_L;
+constexpr int kPairClasses = (kMaxL + 1) * (kMaxL + 2) / 2;
+constexpr int kQuartClasses = kPairClasses * (kPairClasses + 1) / 2;
+
+struct ClassFns {
+    int la, lb, lc, ld;
+    void (*jk)(const pk::JKArgs &, int, hipStream_t, int &);
+    void (*eri)(const pk::EriArgs &, int, hipStream_t);
+};
+
+#define PK_ENTRY(a, b, c, d, shard) ClassFns{a, b, c, d, &pk_launch_jk_##a##b##c##d, &pk_launch_eri_##a##b##c##d},
+// Quartet classes (bra class >= ket class), indexed cb (cb + 1) / 2 + ck with
+// pair classes (0,0) (1,0) (1,1) (2,0) (2,1) (2,2).
+const ClassFns kClasses[kQuartClasses] = {PK_FOR_EACH_CLASS(PK_ENTRY)};
+#undef PK_ENTRY
+
+int quart_class(int cb, int ck) { return cb * (cb + 1) / 2 + ck; }
+int class_la(int c) {
+    int l = 0;
+    while ((l + 1) * (l + 2) / 2 <= c) ++l;
+    return l;
+}
+int class_lb(int c) { return c - class_la(c) * (class_la(c) + 1) / 2; }
+
+// ---------------------------------------------------------------- small kernels
+__global__ void k_scale_density(const double *__restrict__ P, const double *__restrict__ n, double *__restrict__ Pn,
+                                int nb) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)nb * nb) return;
+    const int r = (int)(i / nb), c = (int)(i % nb);
+    Pn[i] = n[r] * n[c] * P[i];
+}
+
+__global__ void k_fill(double *__restrict__ x, size_t n, double v) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = v;
+}
+
+// J = n n^T o (A + A^T), K = n n^T o (B + B^T); optional G = J - K/2
+__global__ void k_finalize(const double *__restrict__ A, const double *__restrict__ B, const double *__restrict__ n,
+                           double *J, double *K, double *G, int nb) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)nb * nb) return;
+    const int r = (int)(i / nb), c = (int)(i % nb);
+    const size_t t = (size_t)c * nb + r;
+    const double s = n[r] * n[c];
+    const double j = s * (A[i] + A[t]);
+    const double k = s * (B[i] + B[t]);
+    if (J) J[i] = j;
+    if (K) K[i] = k;
+    if (G) G[i] = j - 0.5 * k;
+}
+
+struct ClassWork {
+    int cls = 0;
+    int nwaves = 0;
+    int64_t nquart = 0;
+    int64_t nprimq = 0;
+    double flops = 0.0;
+    int *d_wave = nullptr;  // [3][nwaves]: bra, start, count
+    int *d_kets = nullptr;
+};
+
+template <typename T>
+int dev_upload(T **dst, const std::vector<T> &src, hipStream_t s) {
+    *dst = nullptr;
+    if (src.empty()) return PK_OK;
+    PK_HIP(hipMalloc((void **)dst, src.size() * sizeof(T)));
+    PK_HIP(hipMemcpyAsync(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    return PK_OK;
+}
+
+}  // namespace
+
+struct pk_ctx {
+    int nb = 0, ns = 0;
+    std::vector<int> sh_l, sh_np, sh_poff, sh_f0;
+    std::vector<double> sh_c, ex, co, pn;
+    std::vector<double> fnorm;
+    std::vector<pk::PairDev> pairs;
+    std::vector<int> pair_cls;
+    std::vector<double> pair_q;
+    std::vector<pk::PrimDev> prims;
+    std::vector<double> Q;  // nb*nb, normalised functions
+    pk::PairDev *d_pairs = nullptr;
+    pk::PrimDev *d_prims = nullptr;
+    double *d_Q = nullptr, *d_fnorm = nullptr, *d_P = nullptr, *d_Pn = nullptr, *d_A = nullptr, *d_B = nullptr;
+    double *d_J = nullptr, *d_K = nullptr;
+    std::vector<ClassWork> work;
+    bool work_ok = false;
+    int engine = PK_ENGINE_OS;
+    double tol = 1e-10;
+    int precision = PK_PREC_FP64;
+    double cutoff = 0.0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    pk_stats st{};
+};
+
+namespace {
+
+void free_work(pk_ctx *c) {
+    for (ClassWork &w : c->work) {
+        if (w.d_wave) (void)hipFree(w.d_wave);
+        if (w.d_kets) (void)hipFree(w.d_kets);
+    }
+    c->work.clear();
+    c->work_ok = false;
+}
+
+// Raw contracted ERIs (no component norms) of the quartets (bra[i] | ket[i]),
+// all of class (cb, ck) with cb >= ck.  out gets NA NB NC ND values per quartet.
+int run_eri_class(pk_ctx *c, int cb, int ck, const std::vector<int> &bra, const std::vector<int> &ket,
+                  std::vector<double> &out) {
+    const ClassFns &f = kClasses[quart_class(cb, ck)];
+    const int nabcd = pk::ncart(f.la) * pk::ncart(f.lb) * pk::ncart(f.lc) * pk::ncart(f.ld);
+    const int nq = (int)bra.size();
+    out.assign((size_t)nq * nabcd, 0.0);
+    if (nq == 0) return PK_OK;
+    int *d_b = nullptr, *d_k = nullptr;
+    double *d_o = nullptr;
+    int rc;
+    if ((rc = dev_upload(&d_b, bra, c->stream)) != PK_OK) return rc;
+    if ((rc = dev_upload(&d_k, ket, c->stream)) != PK_OK) return rc;
+    PK_HIP(hipMalloc((void **)&d_o, out.size() * sizeof(double)));
+    pk::EriArgs a{d_b, d_k, c->d_pairs, c->d_prims, d_o, nq};
+    f.eri(a, (nq + 255) / 256, c->stream);
+    PK_HIP(hipGetLastError());
+    PK_HIP(hipMemcpyAsync(out.data(), d_o, out.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    PK_HIP(hipStreamSynchronize(c->stream));
+    (void)hipFree(d_b);
+    (void)hipFree(d_k);
+    (void)hipFree(d_o);
+    return PK_OK;
+}
+
+int build_schwarz(pk_ctx *c) {
+    const int nb = c->nb;
+    c->Q.assign((size_t)nb * nb, 0.0);
+    for (int cls = 0; cls < kPairClasses; ++cls) {
+        std::vector<int> idx;
+        for (int p = 0; p < (int)c->pairs.size(); ++p)
+            if (c->pair_cls[p] == cls) idx.push_back(p);
+        if (idx.empty()) continue;
+        std::vector<double> out;
+        int rc = run_eri_class(c, cls, cls, idx, idx, out);
+        if (rc != PK_OK) return rc;
+        const int la = class_la(cls), lb = class_lb(cls);
+        const int na = pk::ncart(la), nbb = pk::ncart(lb), nab = na * nbb;
+        for (size_t q = 0; q < idx.size(); ++q) {
+            const pk::PairDev &p = c->pairs[idx[q]];
+            double qmax = 0.0;
+            for (int ia = 0; ia < na; ++ia)
+                for (int ib = 0; ib < nbb; ++ib) {
+                    const int i = p.f0 + ia, j = p.f1 + ib;
+                    const double raw = out[q * (size_t)nab * nab + (size_t)(ia * nbb + ib) * nab + ia * nbb + ib];
+                    const double v = c->fnorm[i] * c->fnorm[j] * std::sqrt(std::fabs(raw));
+                    c->Q[(size_t)i * nb + j] = c->Q[(size_t)j * nb + i] = v;
+                    qmax = std::max(qmax, v);
+                }
+            c->pair_q[idx[q]] = qmax;
+        }
+    }
+    return PK_OK;
+}
+
+// Screened work lists: for every quartet class (cb >= ck) and every bra pair,
+// the ket pairs with qmax_bra * qmax_ket >= tol (and ket <= bra when cb == ck),
+// sorted by primitive count, cut into waves of at most 64 kets.
+int build_work(pk_ctx *c) {
+    free_work(c);
+    const int np = (int)c->pairs.size();
+    std::vector<std::vector<int>> by_cls(kPairClasses);
+    for (int p = 0; p < np; ++p) by_cls[c->pair_cls[p]].push_back(p);
+    for (auto &v : by_cls)
+        std::stable_sort(v.begin(), v.end(), [&](int x, int y) { return c->pair_q[x] > c->pair_q[y]; });
+    c->st.quartets = 0;
+    c->st.primitive_quartets = 0;
+    c->st.model_flops = 0.0;
+    for (int cb = 0; cb < kPairClasses; ++cb)
+        for (int ck = 0; ck <= cb; ++ck) {
+            ClassWork w;
+            w.cls = quart_class(cb, ck);
+            const ClassFns &f = kClasses[w.cls];
+            const std::vector<int> &B = by_cls[cb], &Kl = by_cls[ck];
+            if (B.empty() || Kl.empty()) continue;
+            std::vector<int> wb, ws, wc, kets;
+            std::vector<int> tmp;
+            for (size_t ib = 0; ib < B.size(); ++ib) {
+                const int b = B[ib];
+                const double qb = c->pair_q[b];
+                // number of kets in q-descending order passing the test
+                size_t lo = 0, hi = Kl.size();
+                while (lo < hi) {
+                    const size_t mid = (lo + hi) / 2;
+                    if (qb * c->pair_q[Kl[mid]] >= c->tol)
+                        lo = mid + 1;
+                    else
+                        hi = mid;
+                }
+                size_t n = lo;
+                if (cb == ck) n = std::min(n, ib + 1);
+                if (n == 0) continue;
+                tmp.assign(Kl.begin(), Kl.begin() + n);
+                std::stable_sort(tmp.begin(), tmp.end(),
+                                 [&](int x, int y) { return c->pairs[x].nprim > c->pairs[y].nprim; });
+                const int kb = c->pairs[b].nprim;
+                for (size_t s = 0; s < n; s += 64) {
+                    wb.push_back(b);
+                    ws.push_back((int)kets.size() + (int)s);
+                    wc.push_back((int)std::min<size_t>(64, n - s));
+                }
+                for (size_t s = 0; s < n; ++s) {
+                    const double K = (double)kb * c->pairs[tmp[s]].nprim;
+                    w.nprimq += (int64_t)K;
+                    w.flops += pk::model_flops(f.la, f.lb, f.lc, f.ld, K);
+                }
+                kets.insert(kets.end(), tmp.begin(), tmp.end());
+                w.nquart += (int64_t)n;
+            }
+            if (w.nquart == 0) continue;
+            w.nwaves = (int)wb.size();
+            std::vector<int> packed;
+            packed.reserve(3 * wb.size());
+            packed.insert(packed.end(), wb.begin(), wb.end());
+            packed.insert(packed.end(), ws.begin(), ws.end());
+            packed.insert(packed.end(), wc.begin(), wc.end());
+            int rc;
+            if ((rc = dev_upload(&w.d_wave, packed, c->stream)) != PK_OK) return rc;
+            if ((rc = dev_upload(&w.d_kets, kets, c->stream)) != PK_OK) return rc;
+            c->st.quartets += w.nquart;
+            c->st.primitive_quartets += w.nprimq;
+            c->st.model_flops += w.flops;
+            c->work.push_back(w);
+        }
+    // largest classes first
+    std::sort(c->work.begin(), c->work.end(), [](const ClassWork &x, const ClassWork &y) { return x.flops > y.flops; });
+    PK_HIP(hipStreamSynchronize(c->stream));
+    c->work_ok = true;
+    return PK_OK;
+}
+
+int ensure_buffers(pk_ctx *c) {
+    const size_t n2 = (size_t)c->nb * c->nb * sizeof(double);
+    if (!c->d_P) PK_HIP(hipMalloc((void **)&c->d_P, n2));
+    if (!c->d_Pn) PK_HIP(hipMalloc((void **)&c->d_Pn, n2));
+    if (!c->d_A) PK_HIP(hipMalloc((void **)&c->d_A, n2));
+    if (!c->d_B) PK_HIP(hipMalloc((void **)&c->d_B, n2));
+    if (!c->d_J) PK_HIP(hipMalloc((void **)&c->d_J, n2));
+    if (!c->d_K) PK_HIP(hipMalloc((void **)&c->d_K, n2));
+    return PK_OK;
+}
+
+// One build: P (device) -> A, B accumulators -> J, K, G (device, any may be null)
+int build_device(pk_ctx *c, const double *dP, double *dJ, double *dK, double *dG) {
+    int rc;
+    if (!c->work_ok && (rc = build_work(c)) != PK_OK) return rc;
+    if ((rc = ensure_buffers(c)) != PK_OK) return rc;
+    const int nb = c->nb;
+    const size_t n2 = (size_t)nb * nb;
+    const int g2 = (int)((n2 + 255) / 256);
+    hipStream_t s = c->stream;
+    PK_HIP(hipEventRecord(c->ev0, s));
+    hipLaunchKernelGGL(k_scale_density, dim3(g2), dim3(256), 0, s, dP, c->d_fnorm, c->d_Pn, nb);
+    hipLaunchKernelGGL(k_fill, dim3(g2), dim3(256), 0, s, c->d_A, n2, 0.0);
+    hipLaunchKernelGGL(k_fill, dim3(g2), dim3(256), 0, s, c->d_B, n2, 0.0);
+    int nl = 3;
+    for (const ClassWork &w : c->work) {
+        const int *wv = w.d_wave;
+        pk::JKArgs a{wv, wv + w.nwaves, wv + 2 * w.nwaves, w.d_kets, c->d_pairs, c->d_prims,
+                     c->d_Pn, c->d_Q, c->d_A, c->d_B, c->tol, nb, w.nwaves};
+        kClasses[w.cls].jk(a, (w.nwaves + 3) / 4, s, nl);
+    }
+    hipLaunchKernelGGL(k_finalize, dim3(g2), dim3(256), 0, s, c->d_A, c->d_B, c->d_fnorm, dJ, dK, dG, nb);
+    ++nl;
+    PK_HIP(hipGetLastError());
+    PK_HIP(hipEventRecord(c->ev1, s));
+    c->st.kernel_launches = nl;
+    return PK_OK;
+}
+
+int finish_timing(pk_ctx *c) {
+    PK_HIP(hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    PK_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->st.last_build_ms = ms;
+    c->st.last_eri_kernel_ms = ms;
+    return PK_OK;
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+extern "C" {
+
+const char *pk_last_error(void) { return g_err.c_str(); }
+
+int pk_ctx_create(const pk_basis_desc *d, pk_ctx **out) {
+    if (!d || !out || d->nshells <= 0 || !d->center || !d->l || !d->nprim || !d->exponent || !d->coef || !d->prim_norm)
+        return fail(PK_ERR_ARG, "pk_ctx_create: null or empty basis description");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(PK_ERR_HIP, "pk_ctx_create: no HIP device visible");
+    pk_ctx *c = new pk_ctx;
+    c->ns = d->nshells;
+    int off = 0, f0 = 0;
+    for (int s = 0; s < c->ns; ++s) {
+        const int l = d->l[s], np = d->nprim[s];
+        if (l < 0 || l > kMaxL) {
+            delete c;
+            return fail(PK_ERR_UNSUPPORTED, "pk_ctx_create: shell " + std::to_string(s) + " has l=" +
+                                                std::to_string(l) + "; this build supports l <= " +
+                                                std::to_string(kMaxL));
+        }
+        if (np <= 0) {
+            delete c;
+            return fail(PK_ERR_ARG, "pk_ctx_create: shell with no primitives");
+        }
+        c->sh_l.push_back(l);
+        c->sh_np.push_back(np);
+        c->sh_poff.push_back(off);
+        c->sh_f0.push_back(f0);
+        for (int q = 0; q < 3; ++q) c->sh_c.push_back(d->center[3 * s + q]);
+        for (int k = 0; k < np; ++k) {
+            c->ex.push_back(d->exponent[off + k]);
+            c->co.push_back(d->coef[off + k]);
+            c->pn.push_back(d->prim_norm[off + k]);
+        }
+        // component norms 1/sqrt((2lx-1)!!(2ly-1)!!(2lz-1)!!)  (gaussian.cpp:286-298)
+        for (int k = 0; k < pk::ncart(l); ++k) {
+            auto df = [](int n) {
+                int r = 1;
+                while (n > 0) {
+                    r *= n;
+                    n -= 2;
+                }
+                return r;
+            };
+            const int x = pk::cart_x(l, k), y = pk::cart_y(l, k), z = pk::cart_z(l, k);
+            c->fnorm.push_back(1.0 / std::sqrt((double)(df(2 * x - 1) * df(2 * y - 1) * df(2 * z - 1))));
+        }
+        off += np;
+        f0 += pk::ncart(l);
+    }
+    c->nb = f0;
+    // shell pairs a >= b, first shell carries the larger l (types.h:606-675 pair data)
+    for (int A = 0; A < c->ns; ++A)
+        for (int Bs = 0; Bs <= A; ++Bs) {
+            const int s0 = c->sh_l[A] >= c->sh_l[Bs] ? A : Bs, s1 = s0 == A ? Bs : A;
+            pk::PairDev p{};
+            p.f0 = c->sh_f0[s0];
+            p.f1 = c->sh_f0[s1];
+            p.diag = A == Bs;
+            p.prim_off = (int)c->prims.size();
+            p.nprim = c->sh_np[s0] * c->sh_np[s1];
+            const double *Ac = &c->sh_c[3 * s0], *Bc = &c->sh_c[3 * s1];
+            for (int q = 0; q < 3; ++q) p.ab[q] = Ac[q] - Bc[q];
+            const double R2 = p.ab[0] * p.ab[0] + p.ab[1] * p.ab[1] + p.ab[2] * p.ab[2];
+            for (int i = 0; i < c->sh_np[s0]; ++i)
+                for (int j = 0; j < c->sh_np[s1]; ++j) {
+                    const double al = c->ex[c->sh_poff[s0] + i], be = c->ex[c->sh_poff[s1] + j];
+                    const double ca = c->co[c->sh_poff[s0] + i] * c->pn[c->sh_poff[s0] + i];
+                    const double cb = c->co[c->sh_poff[s1] + j] * c->pn[c->sh_poff[s1] + j];
+                    pk::PrimDev pp{};
+                    pp.zeta = al + be;
+                    const double iz = 1.0 / pp.zeta;
+                    pp.kc = std::pow(M_PI * iz, 1.5) * std::exp(-(al * be * iz) * R2) * ca * cb;
+                    for (int q = 0; q < 3; ++q) {
+                        pp.p[q] = (al * Ac[q] + be * Bc[q]) * iz;
+                        pp.pa[q] = pp.p[q] - Ac[q];
+                    }
+                    c->prims.push_back(pp);
+                }
+            c->pairs.push_back(p);
+            c->pair_cls.push_back(pk::pair_class(c->sh_l[s0], c->sh_l[s1]));
+            c->pair_q.push_back(0.0);
+        }
+    c->st.nbasis = c->nb;
+    c->st.nshells = c->ns;
+    c->st.shell_pairs = (int64_t)c->pairs.size();
+    int rc = PK_OK;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        pk_ctx_destroy(c);
+        return fail(PK_ERR_HIP, "pk_ctx_create: stream/event creation failed");
+    }
+    c->own_stream = true;
+    if ((rc = dev_upload(&c->d_pairs, c->pairs, c->stream)) != PK_OK ||
+        (rc = dev_upload(&c->d_prims, c->prims, c->stream)) != PK_OK ||
+        (rc = dev_upload(&c->d_fnorm, c->fnorm, c->stream)) != PK_OK || (rc = build_schwarz(c)) != PK_OK ||
+        (rc = dev_upload(&c->d_Q, c->Q, c->stream)) != PK_OK) {
+        const std::string msg = g_err;
+        pk_ctx_destroy(c);
+        return fail(rc, msg);
+    }
+    if (hipStreamSynchronize(c->stream) != hipSuccess) {
+        pk_ctx_destroy(c);
+        return fail(PK_ERR_HIP, "pk_ctx_create: device error during setup");
+    }
+    *out = c;
+    return PK_OK;
+}
+
+int pk_ctx_destroy(pk_ctx *c) {
+    if (!c) return PK_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_work(c);
+    for (void *p : {(void *)c->d_pairs, (void *)c->d_prims, (void *)c->d_Q, (void *)c->d_fnorm, (void *)c->d_P,
+                    (void *)c->d_Pn, (void *)c->d_A, (void *)c->d_B, (void *)c->d_J, (void *)c->d_K})
+        if (p) (void)hipFree(p);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return PK_OK;
+}
+
+int pk_ctx_set(pk_ctx *c, int engine, double tol, int precision, double cutoff) {
+    if (!c) return fail(PK_ERR_ARG, "pk_ctx_set: null context");
+    if (engine < PK_ENGINE_OS || engine > PK_ENGINE_AUTO) return fail(PK_ERR_ARG, "pk_ctx_set: unknown engine");
+    if (!(tol >= 0.0)) return fail(PK_ERR_ARG, "pk_ctx_set: tol_eri must be >= 0");
+    if (precision != PK_PREC_FP64)
+        return fail(PK_ERR_UNSUPPORTED, "pk_ctx_set: only PK_PREC_FP64 is available in this build");
+    c->engine = engine;
+    c->precision = precision;
+    c->cutoff = cutoff;
+    if (tol != c->tol) {
+        c->tol = tol;
+        c->work_ok = false;
+    }
+    return PK_OK;
+}
+
+int pk_ctx_set_stream(pk_ctx *c, void *stream) {
+    if (!c) return fail(PK_ERR_ARG, "pk_ctx_set_stream: null context");
+    (void)hipStreamSynchronize(c->stream);
+    if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+    if (stream) {
+        c->stream = (hipStream_t)stream;
+        c->own_stream = false;
+    } else {
+        PK_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        c->own_stream = true;
+    }
+    return PK_OK;
+}
+
+int pk_ctx_nbasis(const pk_ctx *c) { return c ? c->nb : -1; }
+
+int pk_ctx_stats(const pk_ctx *c, pk_stats *out) {
+    if (!c || !out) return fail(PK_ERR_ARG, "pk_ctx_stats: null argument");
+    *out = c->st;
+    return PK_OK;
+}
+
+static int fock_common(pk_ctx *c, const double *P, double *J, double *K, double *G, int memkind) {
+    if (!c || !P) return fail(PK_ERR_ARG, "pk_fock: null context or density");
+    if (memkind != PK_MEM_HOST && memkind != PK_MEM_DEVICE) return fail(PK_ERR_ARG, "pk_fock: bad memkind");
+    int rc;
+    if ((rc = ensure_buffers(c)) != PK_OK) return rc;
+    const size_t n2 = (size_t)c->nb * c->nb * sizeof(double);
+    if (memkind == PK_MEM_DEVICE) {
+        if ((rc = build_device(c, P, J, K, G)) != PK_OK) return rc;
+        return finish_timing(c);
+    }
+    PK_HIP(hipMemcpyAsync(c->d_P, P, n2, hipMemcpyHostToDevice, c->stream));
+    double *dJ = J ? c->d_J : nullptr, *dK = K ? c->d_K : nullptr, *dG = G ? c->d_J : nullptr;
+    if ((rc = build_device(c, c->d_P, dJ, dK, dG)) != PK_OK) return rc;
+    if (J) PK_HIP(hipMemcpyAsync(J, c->d_J, n2, hipMemcpyDeviceToHost, c->stream));
+    if (K) PK_HIP(hipMemcpyAsync(K, c->d_K, n2, hipMemcpyDeviceToHost, c->stream));
+    if (G) PK_HIP(hipMemcpyAsync(G, c->d_J, n2, hipMemcpyDeviceToHost, c->stream));
+    PK_HIP(hipStreamSynchronize(c->stream));
+    return finish_timing(c);
+}
+
+int pk_fock_jk(pk_ctx *c, const double *P, double *J, double *K, int memkind) {
+    return fock_common(c, P, J, K, nullptr, memkind);
+}
+
+int pk_fock_2e(pk_ctx *c, const double *P, double *G, int memkind) {
+    if (!G) return fail(PK_ERR_ARG, "pk_fock_2e: null G");
+    return fock_common(c, P, nullptr, nullptr, G, memkind);
+}
+
+int pk_fock_jk_uhf(pk_ctx *c, const double *Pa, const double *Pb, double *Jt, double *Ka, double *Kb, int memkind) {
+    if (!c || !Pa || !Pb) return fail(PK_ERR_ARG, "pk_fock_jk_uhf: null argument");
+    if (memkind != PK_MEM_HOST) return fail(PK_ERR_UNSUPPORTED, "pk_fock_jk_uhf: host pointers only in this build");
+    const size_t n2 = (size_t)c->nb * c->nb;
+    std::vector<double> Ja(n2), Jb(n2), ka(n2), kb(n2);
+    int rc;
+    if ((rc = pk_fock_jk(c, Pa, Ja.data(), ka.data(), PK_MEM_HOST)) != PK_OK) return rc;
+    if ((rc = pk_fock_jk(c, Pb, Jb.data(), kb.data(), PK_MEM_HOST)) != PK_OK) return rc;
+    for (size_t i = 0; i < n2; ++i) {
+        if (Jt) Jt[i] = Ja[i] + Jb[i];
+        if (Ka) Ka[i] = ka[i];
+        if (Kb) Kb[i] = kb[i];
+    }
+    return PK_OK;
+}
+
+int pk_schwarz(pk_ctx *c, double *Q, int memkind) {
+    if (!c || !Q) return fail(PK_ERR_ARG, "pk_schwarz: null argument");
+    const size_t n2 = (size_t)c->nb * c->nb * sizeof(double);
+    if (memkind == PK_MEM_DEVICE) {
+        PK_HIP(hipMemcpyAsync(Q, c->d_Q, n2, hipMemcpyDeviceToDevice, c->stream));
+        PK_HIP(hipStreamSynchronize(c->stream));
+    } else {
+        std::memcpy(Q, c->Q.data(), n2);
+    }
+    return PK_OK;
+}
+
+int pk_eri_tensor(pk_ctx *c, double *out) {
+    if (!c || !out) return fail(PK_ERR_ARG, "pk_eri_tensor: null argument");
+    const size_t nb = c->nb, nb2 = nb * nb, nb3 = nb2 * nb;
+    if ((double)nb3 * nb * 8.0 > 2147483648.0) return fail(PK_ERR_ARG, "pk_eri_tensor: nb^4 tensor exceeds 2 GiB");
+    std::fill(out, out + nb3 * nb, 0.0);
+    const int np = (int)c->pairs.size();
+    std::vector<std::vector<int>> qb(kQuartClasses), qk(kQuartClasses);
+    for (int p = 0; p < np; ++p)
+        for (int q = 0; q <= p; ++q) {
+            int b = p, k = q;
+            if (c->pair_cls[b] < c->pair_cls[k]) std::swap(b, k);
+            const int cls = quart_class(c->pair_cls[b], c->pair_cls[k]);
+            qb[cls].push_back(b);
+            qk[cls].push_back(k);
+        }
+    for (int cls = 0; cls < kQuartClasses; ++cls) {
+        if (qb[cls].empty()) continue;
+        const ClassFns &f = kClasses[cls];
+        std::vector<double> v;
+        int rc = run_eri_class(c, pk::pair_class(f.la, f.lb), pk::pair_class(f.lc, f.ld), qb[cls], qk[cls], v);
+        if (rc != PK_OK) return rc;
+        const int na = pk::ncart(f.la), nbs = pk::ncart(f.lb), nc = pk::ncart(f.lc), nd = pk::ncart(f.ld);
+        for (size_t q = 0; q < qb[cls].size(); ++q) {
+            const pk::PairDev &B = c->pairs[qb[cls][q]], &K = c->pairs[qk[cls][q]];
+            for (int ia = 0; ia < na; ++ia)
+                for (int ib = 0; ib < nbs; ++ib)
+                    for (int ic = 0; ic < nc; ++ic)
+                        for (int id = 0; id < nd; ++id) {
+                            const size_t i = B.f0 + ia, j = B.f1 + ib, k = K.f0 + ic, l = K.f1 + id;
+                            if (c->Q[i * nb + j] * c->Q[k * nb + l] < c->tol) continue;
+                            const double val = c->fnorm[i] * c->fnorm[j] * c->fnorm[k] * c->fnorm[l] *
+                                               v[q * (size_t)na * nbs * nc * nd + ((ia * nbs + ib) * nc + ic) * nd + id];
+                            const size_t idx[8] = {i * nb3 + j * nb2 + k * nb + l, j * nb3 + i * nb2 + k * nb + l,
+                                                   i * nb3 + j * nb2 + l * nb + k, j * nb3 + i * nb2 + l * nb + k,
+                                                   k * nb3 + l * nb2 + i * nb + j, l * nb3 + k * nb2 + i * nb + j,
+                                                   k * nb3 + l * nb2 + j * nb + i, l * nb3 + k * nb2 + j * nb + i};
+                            for (size_t x : idx) out[x] = val;
+                        }
+        }
+    }
+    return PK_OK;
+}
+
+}  // extern "C"

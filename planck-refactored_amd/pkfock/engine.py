@@ -1,0 +1,168 @@
+"""ctypes binding of the HIP Fock-build engine (lib/libpkfock.so, include/pk_fock.h).
+
+The engine is the product: there is no CPU fallback.  Loading fails loudly when
+the shared library is missing, and every call raises PkError on a non-zero
+status with the engine's own message.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libpkfock.so")
+
+ENGINE = {"os": 0, "obara-saika": 0, "rys": 1, "rys-quadrature": 1, "auto": 2}
+PREC_FP64, PREC_MIXED = 0, 1
+MEM_HOST, MEM_DEVICE = 0, 1
+
+
+class PkError(RuntimeError):
+    pass
+
+
+class BasisDesc(ctypes.Structure):
+    _fields_ = [("nshells", ctypes.c_int32),
+                ("center", ctypes.POINTER(ctypes.c_double)),
+                ("l", ctypes.POINTER(ctypes.c_int32)),
+                ("nprim", ctypes.POINTER(ctypes.c_int32)),
+                ("exponent", ctypes.POINTER(ctypes.c_double)),
+                ("coef", ctypes.POINTER(ctypes.c_double)),
+                ("prim_norm", ctypes.POINTER(ctypes.c_double))]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("nbasis", ctypes.c_int64), ("nshells", ctypes.c_int64), ("shell_pairs", ctypes.c_int64),
+                ("quartets", ctypes.c_int64), ("primitive_quartets", ctypes.c_int64),
+                ("model_flops", ctypes.c_double), ("last_build_ms", ctypes.c_double),
+                ("last_eri_kernel_ms", ctypes.c_double), ("kernel_launches", ctypes.c_int64)]
+
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise PkError("HIP engine not built: %s is missing (run __graft_entry__.build())" % LIB_PATH)
+    lib = ctypes.CDLL(LIB_PATH)
+    vp, dp, ip = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_int
+    lib.pk_ctx_create.argtypes = [ctypes.POINTER(BasisDesc), ctypes.POINTER(vp)]
+    lib.pk_ctx_destroy.argtypes = [vp]
+    lib.pk_ctx_set.argtypes = [vp, ip, ctypes.c_double, ip, ctypes.c_double]
+    lib.pk_ctx_set_stream.argtypes = [vp, vp]
+    lib.pk_ctx_nbasis.argtypes = [vp]
+    lib.pk_ctx_stats.argtypes = [vp, ctypes.POINTER(Stats)]
+    lib.pk_fock_jk.argtypes = [vp, vp, vp, vp, ip]
+    lib.pk_fock_2e.argtypes = [vp, vp, vp, ip]
+    lib.pk_fock_jk_uhf.argtypes = [vp, vp, vp, vp, vp, vp, ip]
+    lib.pk_schwarz.argtypes = [vp, vp, ip]
+    lib.pk_eri_tensor.argtypes = [vp, vp]
+    lib.pk_last_error.restype = ctypes.c_char_p
+    _lib = lib
+    return lib
+
+
+def _check(rc, what):
+    if rc != 0:
+        msg = load().pk_last_error().decode(errors="replace")
+        raise PkError("%s failed (%d): %s" % (what, rc, msg))
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Shells:
+    """Contracted Cartesian shells in basis order (the pk_basis_desc arrays)."""
+
+    def __init__(self, center, l, nprim, exponent, coef, prim_norm):
+        self.center = np.ascontiguousarray(center, dtype=np.float64).reshape(-1, 3)
+        self.l = np.ascontiguousarray(l, dtype=np.int32)
+        self.nprim = np.ascontiguousarray(nprim, dtype=np.int32)
+        self.exponent = np.ascontiguousarray(exponent, dtype=np.float64)
+        self.coef = np.ascontiguousarray(coef, dtype=np.float64)
+        self.prim_norm = np.ascontiguousarray(prim_norm, dtype=np.float64)
+
+    @property
+    def nbasis(self):
+        return int(sum((l + 1) * (l + 2) // 2 for l in self.l))
+
+    def desc(self):
+        d = BasisDesc()
+        d.nshells = len(self.l)
+        dbl = ctypes.POINTER(ctypes.c_double)
+        i32 = ctypes.POINTER(ctypes.c_int32)
+        d.center = self.center.ctypes.data_as(dbl)
+        d.l = self.l.ctypes.data_as(i32)
+        d.nprim = self.nprim.ctypes.data_as(i32)
+        d.exponent = self.exponent.ctypes.data_as(dbl)
+        d.coef = self.coef.ctypes.data_as(dbl)
+        d.prim_norm = self.prim_norm.ctypes.data_as(dbl)
+        return d
+
+
+class FockEngine:
+    """One engine context per geometry (pk_ctx)."""
+
+    def __init__(self, shells, engine="os", tol_eri=1e-10):
+        lib = load()
+        self._keep = shells
+        self.ctx = ctypes.c_void_p()
+        _check(lib.pk_ctx_create(ctypes.byref(shells.desc()), ctypes.byref(self.ctx)), "pk_ctx_create")
+        self.nbasis = lib.pk_ctx_nbasis(self.ctx)
+        self.set(engine, tol_eri)
+
+    def set(self, engine="os", tol_eri=1e-10, precision=PREC_FP64, fp32_cutoff=0.0):
+        e = ENGINE[engine] if isinstance(engine, str) else int(engine)
+        _check(load().pk_ctx_set(self.ctx, e, float(tol_eri), int(precision), float(fp32_cutoff)), "pk_ctx_set")
+
+    def close(self):
+        if self.ctx:
+            load().pk_ctx_destroy(self.ctx)
+            self.ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def stats(self):
+        s = Stats()
+        _check(load().pk_ctx_stats(self.ctx, ctypes.byref(s)), "pk_ctx_stats")
+        return {k: getattr(s, k) for k, _ in Stats._fields_}
+
+    def schwarz(self):
+        Q = np.zeros((self.nbasis, self.nbasis))
+        _check(load().pk_schwarz(self.ctx, _ptr(Q), MEM_HOST), "pk_schwarz")
+        return Q
+
+    def eri_tensor(self):
+        n = self.nbasis
+        out = np.zeros((n, n, n, n))
+        _check(load().pk_eri_tensor(self.ctx, _ptr(out)), "pk_eri_tensor")
+        return out
+
+    def jk(self, P):
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        J = np.zeros_like(P)
+        K = np.zeros_like(P)
+        _check(load().pk_fock_jk(self.ctx, _ptr(P), _ptr(J), _ptr(K), MEM_HOST), "pk_fock_jk")
+        return J, K
+
+    def g(self, P):
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        G = np.zeros_like(P)
+        _check(load().pk_fock_2e(self.ctx, _ptr(P), _ptr(G), MEM_HOST), "pk_fock_2e")
+        return G
+
+    def jk_device(self, P_ptr, J_ptr, K_ptr):
+        """P, J, K are device pointers (ints), e.g. torch tensor .data_ptr()."""
+        _check(load().pk_fock_jk(self.ctx, ctypes.c_void_p(P_ptr), ctypes.c_void_p(J_ptr),
+                                 ctypes.c_void_p(K_ptr), MEM_DEVICE), "pk_fock_jk")
+
+    def set_stream(self, stream_ptr):
+        _check(load().pk_ctx_set_stream(self.ctx, ctypes.c_void_p(stream_ptr)), "pk_ctx_set_stream")
