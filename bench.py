@@ -1,0 +1,166 @@
+#!/usr/bin/env python3
+"""bench.py — J/K Fock builds per second of the MI355X engine.
+
+Workload (one GPU): RHF two-electron Fock build J[P], K[P] of all-trans
+n-C20H42 (idealised geometry: C-C 1.54 A, C-H 1.09 A, tetrahedral angles) in
+the Cartesian cc-pVDZ basis (510 functions, up to (dd|dd) quartets), Schwarz
+threshold 1e-12, FP64 throughout.  A step is one build from a device-resident
+density (seeded random symmetric matrix; the FP64 path has no density-dependent
+screening, so the work does not depend on its values) to device-resident J and
+K.  After the timed steps the full RHF SCF of the same molecule runs once
+(core guess, DIIS 8, tol_energy = tol_density = 1e-10) for the SCF wall time.
+
+Printed: one JSON line with the throughput, the roofline of the dominant ERI
+kernel (flop model v1 over its measured HIP-event time) and the CPU baseline
+(the oracle's reference-faithful Fock build, a bounded sample of bra AO pairs
+on the host cores, scaled to one full build).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "planck-refactored_amd"))
+
+METRIC = "J/K Fock builds/s and SCF wall time at |ΔE|≤1e-6 Eh vs plain FP64 GPU"
+PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector / matrix peak
+
+
+def cpu_baseline(Z, X, seconds, threads):
+    """Oracle Fock build (per-AO-function Obara-Saika, the reference's
+    algorithm) on every stride-th bra AO pair, scaled by the stride."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import numpy as np
+    from refcpu import Ref
+    from pkfock.molecules import ANGSTROM_TO_BOHR
+    ref = Ref("cc-pvdz", Z, X * ANGSTROM_TO_BOHR, engine="os", tol_eri=1e-12)
+    n = ref.nbasis
+    npairs = n * (n + 1) // 2
+    rng = np.random.default_rng(0)
+    A = rng.normal(size=(n, n)) * 0.05
+    P = A + A.T
+    ref.jk_sample(P, stride=npairs, offset=0, nthreads=threads)  # Schwarz table outside the timing
+    stride = 16384
+    for _ in range(3):
+        t = time.perf_counter()
+        _, _, nq = ref.jk_sample(P, stride=stride, offset=1, nthreads=threads)
+        dt = time.perf_counter() - t
+        if dt >= 0.5 * seconds or stride <= 64:
+            break
+        stride = max(64, int(stride * max(dt, 1e-3) / seconds))
+    full = dt * stride
+    return {"value": 1.0 / full, "unit": "builds/s", "cores": threads, "kind": "port",
+            "sample": "oracle J/K over every %d-th of %d bra AO pairs (%.0f AO quartets, %.1f s), "
+                      "scaled x%d to one build" % (stride, npairs, nq, dt, stride)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-scf", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this engine measures one GPU; multi-GPU runs are not part of this build "
+              "(the J/K build shards by quartet task list, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from pkfock.host import Molecule
+    from pkfock.molecules import alkane
+
+    dev = torch.device("cuda:0")
+    Z, X = alkane(20)
+    mol = Molecule(Z, X, "cc-pvdz")
+    t = time.perf_counter()
+    eng = mol.engine("os", tol_eri=1e-12)
+    setup_s = time.perf_counter() - t
+    n = eng.nbasis
+    g = torch.Generator().manual_seed(0)
+    A = torch.randn(n, n, generator=g, dtype=torch.float64) * 0.05
+    P = (A + A.T).contiguous().to(dev)
+    J = torch.empty_like(P)
+    K = torch.empty_like(P)
+    stream = torch.cuda.current_stream(dev)
+    eng.set_stream(stream.cuda_stream)
+    t = time.perf_counter()
+    for _ in range(max(args.warmup, 1)):
+        eng.jk_device(P.data_ptr(), J.data_ptr(), K.data_ptr())
+    torch.cuda.synchronize()
+    warm_s = time.perf_counter() - t
+    print("bench: C20H42/cc-pVDZ nb=%d setup %.2fs warmup %.2fs" % (n, setup_s, warm_s), file=sys.stderr, flush=True)
+
+    eng.profile(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        eng.jk_device(P.data_ptr(), J.data_ptr(), K.data_ptr())
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    eng.profile(False)
+    elapsed = t1 - t0
+    ms = elapsed / args.steps * 1e3
+    st = eng.stats()
+    cls = eng.class_stats()
+    for c in cls:
+        c["ms_avg"] = c["ms_total"] / max(c["builds"], 1)
+    dom = max(cls, key=lambda c: c["ms_avg"])
+    achieved = dom["model_flops"] / (dom["ms_avg"] * 1e-3) * 1e-12
+    print("bench: %d steps %.2f ms/step; model %.3e flop/build = %.2f TFLOP/s whole build" % (
+        args.steps, ms, st["model_flops"], st["model_flops"] / ms * 1e-9), file=sys.stderr, flush=True)
+    for c in sorted(cls, key=lambda c: -c["ms_avg"])[:8]:
+        print("  class (%d%d|%d%d) x%d kernels: %8.3f ms  %.3e quartets  %.2f TFLOP/s" % (
+            c["la"], c["lb"], c["lc"], c["ld"], c["kernels"], c["ms_avg"], c["quartets"],
+            c["model_flops"] / (c["ms_avg"] * 1e-3) * 1e-12), file=sys.stderr, flush=True)
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_c20h42_ccpvdz.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            tr = json.load(f)
+        traffic = tr.get(dom_name(dom))
+
+    scf = None
+    if not args.no_scf:
+        r = mol.rhf(engine="os", tol_eri=1e-12)
+        scf = {"wall_s": round(r["wall_s"], 3), "iterations": r["iterations"], "converged": bool(r["converged"]),
+               "energy": r["energy"], "fock_ms_total": round(r["fock_ms"], 1)}
+        print("bench: SCF %s" % scf, file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+        try:
+            cpu = cpu_baseline(Z, X, args.cpu_seconds, threads)
+        except Exception as e:  # the baseline is reported, never required
+            print("bench: cpu baseline unavailable: %s" % e, file=sys.stderr)
+
+    out = {
+        "metric": METRIC, "value": args.steps / elapsed, "unit": "builds/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "strong",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "RHF J/K build, all-trans n-C20H42 (C-C 1.54 A, C-H 1.09 A, tetrahedral), "
+                               "cc-pVDZ Cartesian (510 functions), Schwarz 1e-12, FP64",
+                   "nbasis": n, "shell_quartets": st["quartets"], "primitive_quartets": st["primitive_quartets"],
+                   "model_flops_per_build": st["model_flops"],
+                   "whole_build_tflops": st["model_flops"] / ms * 1e-9},
+        "roofline": {"bound": "valu", "kernel": dom_name(dom), "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
+                     "kernel_ms": dom["ms_avg"], "kernel_model_flops": dom["model_flops"]},
+        "cpu_baseline": cpu,
+        "scf": scf,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def dom_name(c):
+    return "k_jk<%d,%d,%d,%d>" % (c["la"], c["lb"], c["lc"], c["ld"])
+
+
+if __name__ == "__main__":
+    main()

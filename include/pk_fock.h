@@ -113,6 +113,21 @@ int pk_schwarz(pk_ctx *ctx, double *Q, int memkind);
  * when nb^4 * 8 bytes exceeds 2 GiB. */
 int pk_eri_tensor(pk_ctx *ctx, double *out);
 
+/* Per-class device timing (HIP events around each class's kernels on the
+ * context stream).  Disabled by default; enabling resets the counters. */
+typedef struct pk_class_stats {
+    int32_t la, lb, lc, ld;       /* angular class of the kernels */
+    int32_t kernels;              /* kernels per build (ket-component chunks) */
+    int64_t quartets;
+    int64_t primitive_quartets;
+    double model_flops;           /* flop model v1 of one build of this class */
+    double ms_total;              /* summed device time over profiled builds */
+    int64_t builds;               /* profiled builds */
+} pk_class_stats;
+int pk_ctx_profile(pk_ctx *ctx, int enable);
+int pk_ctx_class_count(const pk_ctx *ctx);
+int pk_ctx_class_stats(const pk_ctx *ctx, int index, pk_class_stats *out);
+
 const char *pk_last_error(void);
 
 #ifdef __cplusplus

@@ -104,6 +104,10 @@ struct ClassWork {
     double flops = 0.0;
     int *d_wave = nullptr;  // [3][nwaves]: bra, start, count
     int *d_kets = nullptr;
+    int kernels = 0;
+    double ms_total = 0.0;
+    int64_t builds = 0;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
 };
 
 template <typename T>
@@ -140,6 +144,7 @@ struct pk_ctx {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool profile = false;
     pk_stats st{};
 };
 
@@ -149,6 +154,8 @@ void free_work(pk_ctx *c) {
     for (ClassWork &w : c->work) {
         if (w.d_wave) (void)hipFree(w.d_wave);
         if (w.d_kets) (void)hipFree(w.d_kets);
+        if (w.e0) (void)hipEventDestroy(w.e0);
+        if (w.e1) (void)hipEventDestroy(w.e1);
     }
     c->work.clear();
     c->work_ok = false;
@@ -311,11 +318,19 @@ int build_device(pk_ctx *c, const double *dP, double *dJ, double *dK, double *dG
     hipLaunchKernelGGL(k_fill, dim3(g2), dim3(256), 0, s, c->d_A, n2, 0.0);
     hipLaunchKernelGGL(k_fill, dim3(g2), dim3(256), 0, s, c->d_B, n2, 0.0);
     int nl = 3;
-    for (const ClassWork &w : c->work) {
+    for (ClassWork &w : c->work) {
         const int *wv = w.d_wave;
         pk::JKArgs a{wv, wv + w.nwaves, wv + 2 * w.nwaves, w.d_kets, c->d_pairs, c->d_prims,
                      c->d_Pn, c->d_Q, c->d_A, c->d_B, c->tol, nb, w.nwaves};
+        if (c->profile) {
+            if (!w.e0) PK_HIP(hipEventCreate(&w.e0));
+            if (!w.e1) PK_HIP(hipEventCreate(&w.e1));
+            PK_HIP(hipEventRecord(w.e0, s));
+        }
+        const int before = nl;
         kClasses[w.cls].jk(a, (w.nwaves + 3) / 4, s, nl);
+        w.kernels = nl - before;
+        if (c->profile) PK_HIP(hipEventRecord(w.e1, s));
     }
     hipLaunchKernelGGL(k_finalize, dim3(g2), dim3(256), 0, s, c->d_A, c->d_B, c->d_fnorm, dJ, dK, dG, nb);
     ++nl;
@@ -331,6 +346,17 @@ int finish_timing(pk_ctx *c) {
     PK_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->st.last_build_ms = ms;
     c->st.last_eri_kernel_ms = ms;
+    if (c->profile) {
+        double eri = 0.0;
+        for (ClassWork &w : c->work) {
+            float m = 0.f;
+            PK_HIP(hipEventElapsedTime(&m, w.e0, w.e1));
+            w.ms_total += m;
+            w.builds += 1;
+            eri += m;
+        }
+        c->st.last_eri_kernel_ms = eri;
+    }
     return PK_OK;
 }
 
@@ -493,6 +519,36 @@ int pk_ctx_set_stream(pk_ctx *c, void *stream) {
 }
 
 int pk_ctx_nbasis(const pk_ctx *c) { return c ? c->nb : -1; }
+
+int pk_ctx_profile(pk_ctx *c, int enable) {
+    if (!c) return fail(PK_ERR_ARG, "pk_ctx_profile: null context");
+    c->profile = enable != 0;
+    if (c->profile)
+        for (ClassWork &w : c->work) {
+            w.ms_total = 0.0;
+            w.builds = 0;
+        }
+    return PK_OK;
+}
+
+int pk_ctx_class_count(const pk_ctx *c) { return c ? (int)c->work.size() : -1; }
+
+int pk_ctx_class_stats(const pk_ctx *c, int i, pk_class_stats *o) {
+    if (!c || !o || i < 0 || i >= (int)c->work.size()) return fail(PK_ERR_ARG, "pk_ctx_class_stats: bad index");
+    const ClassWork &w = c->work[i];
+    const ClassFns &f = kClasses[w.cls];
+    o->la = f.la;
+    o->lb = f.lb;
+    o->lc = f.lc;
+    o->ld = f.ld;
+    o->kernels = w.kernels;
+    o->quartets = w.nquart;
+    o->primitive_quartets = w.nprimq;
+    o->model_flops = w.flops;
+    o->ms_total = w.ms_total;
+    o->builds = w.builds;
+    return PK_OK;
+}
 
 int pk_ctx_stats(const pk_ctx *c, pk_stats *out) {
     if (!c || !out) return fail(PK_ERR_ARG, "pk_ctx_stats: null argument");

@@ -1,0 +1,399 @@
+// pk_host.cpp — basis construction, one-electron integrals and nuclear
+// repulsion for the host side of the engine (include/pk_scf.h).
+//
+// Reference behaviour followed:
+//   Gaussian94 reader, SP split, D exponents ........ src/basis/gaussian.cpp:54-163
+//   per-atom shells in file order, Nc scaling ...... src/basis/gaussian.cpp:183-302
+//   primitive / contracted normalisation ........... src/basis/basis.cpp:39-75
+//   Cartesian component order and norms ............ src/basis/basis.cpp:20-37, gaussian.cpp:286-298
+//   overlap / kinetic ............................... src/integrals/os.cpp:248-395
+//   nuclear attraction .............................. src/integrals/os.cpp:462-602
+//   nuclear repulsion ............................... src/base/types.h:934-964
+// The one-electron integrals are built per shell pair with all Cartesian
+// components at once: 1-D overlap recurrences for S and T, and a Rys rule
+// (the engine's own table) for the nuclear attraction.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "pk_host_internal.h"
+#include "rys_table.inc"
+
+namespace pkh {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &m) {
+    g_err = m;
+    return code;
+}
+
+namespace {
+
+const char *kSym[] = {"",   "H",  "He", "Li", "Be", "B",  "C",  "N",  "O",  "F",  "Ne", "Na", "Mg",
+                      "Al", "Si", "P",  "S",  "Cl", "Ar", "K",  "Ca", "Sc", "Ti", "V",  "Cr", "Mn",
+                      "Fe", "Co", "Ni", "Cu", "Zn", "Ga", "Ge", "As", "Se", "Br", "Kr"};
+constexpr int kMaxZ = 36;
+
+struct RawShell {
+    int l;
+    std::vector<double> e, c;
+};
+
+std::string fortran_exp(std::string s) {
+    for (char &ch : s)
+        if (ch == 'D' || ch == 'd') ch = 'E';
+    return s;
+}
+
+int shell_l(const std::string &lab) {
+    static const char *L = "SPDFGH";
+    if (lab.size() != 1) return -1;
+    const char *p = std::strchr(L, lab[0]);
+    return p ? (int)(p - L) : -1;
+}
+
+bool parse_gbs(const char *path, std::map<std::string, std::vector<RawShell>> &out) {
+    std::ifstream f(path);
+    if (!f) {
+        g_err = std::string("cannot open basis file: ") + path;
+        return false;
+    }
+    std::string line, elem;
+    while (std::getline(f, line)) {
+        if (!line.empty() && line.back() == '\r') line.pop_back();
+        if (line.empty() || line[0] == '!') continue;
+        if (line == "****") {
+            elem.clear();
+            continue;
+        }
+        std::istringstream hs(line);
+        std::string tok;
+        int charge = 0;
+        if ((hs >> tok >> charge) && hs.eof()) {
+            elem = tok;
+            out[elem];
+            continue;
+        }
+        if (elem.empty()) {
+            g_err = "basis file: shell before element header";
+            return false;
+        }
+        std::istringstream ss(line);
+        std::string lab;
+        int np = 0;
+        double scale = 1.0;
+        ss >> lab >> np >> scale;
+        const bool sp = lab == "SP";
+        if (!sp && shell_l(lab) < 0) {
+            g_err = "basis file: malformed shell line: " + line;
+            return false;
+        }
+        RawShell a{sp ? 0 : shell_l(lab), {}, {}}, b{1, {}, {}};
+        for (int i = 0; i < np; ++i) {
+            if (!std::getline(f, line)) {
+                g_err = "basis file: truncated shell";
+                return false;
+            }
+            std::istringstream ps(fortran_exp(line));
+            double ex, c1, c2 = 0.0;
+            ps >> ex >> c1;
+            if (sp) ps >> c2;
+            a.e.push_back(ex);
+            a.c.push_back(c1 * scale);
+            if (sp) {
+                b.e.push_back(ex);
+                b.c.push_back(c2 * scale);
+            }
+        }
+        out[elem].push_back(a);
+        if (sp) out[elem].push_back(b);
+    }
+    return true;
+}
+
+int dfac(int n) {
+    int r = 1;
+    for (; n > 0; n -= 2) r *= n;
+    return r;
+}
+
+// host evaluation of the engine's Rys rule (same table as the kernels)
+void rys_rule_host(int n, double T, double *u, double *w) {
+    if (T >= pk_rys_tmax[n]) {
+        const double it = 1.0 / T, sit = std::sqrt(it);
+        for (int r = 0; r < n; ++r) {
+            u[r] = pk_rys_ax2[pk_rys_abase[n] + r] * it;
+            w[r] = pk_rys_aw[pk_rys_abase[n] + r] * sit;
+        }
+        return;
+    }
+    const int i = (int)T;
+    const double x = 2.0 * (T - i) - 1.0, x2 = 2.0 * x;
+    const double *c = pk_rys_coef + pk_rys_base[n] + i * (2 * n * (PK_RYS_DEG + 1));
+    for (int f = 0; f < 2 * n; ++f) {
+        const double *cf = c + f * (PK_RYS_DEG + 1);
+        double b1 = 0, b2 = 0;
+        for (int j = PK_RYS_DEG; j >= 1; --j) {
+            const double t = std::fma(x2, b1, cf[j] - b2);
+            b2 = b1;
+            b1 = t;
+        }
+        const double v = std::fma(x, b1, cf[0] - b2);
+        (f < n ? u[f] : w[f - n]) = v;
+    }
+}
+
+int ncart(int l) { return (l + 1) * (l + 2) / 2; }
+void cart(int l, int k, int &x, int &y, int &z) {
+    int lx = l;
+    while (k > l - lx) {
+        k -= l - lx + 1;
+        --lx;
+    }
+    x = lx;
+    z = k;
+    y = l - lx - k;
+}
+
+}  // namespace
+
+int load_basis(const char *path, int natoms, const int32_t *Z, const double *xyz, Basis &b) {
+    std::map<std::string, std::vector<RawShell>> gbs;
+    if (!parse_gbs(path, gbs)) return PK_ERR_ARG;
+    b = Basis{};
+    b.Z.assign(Z, Z + natoms);
+    b.xyz.assign(xyz, xyz + 3 * natoms);
+    for (int a = 0; a < natoms; ++a) {
+        if (Z[a] < 1 || Z[a] > kMaxZ) return fail(PK_ERR_ARG, "unsupported element Z=" + std::to_string(Z[a]));
+        auto it = gbs.find(kSym[Z[a]]);
+        if (it == gbs.end()) return fail(PK_ERR_ARG, std::string("Unsupported Element: ") + kSym[Z[a]]);
+        for (const RawShell &r : it->second) {
+            const int l = r.l, np = (int)r.e.size();
+            const double pi15 = std::pow(M_PI, 1.5);
+            std::vector<double> nrm(np);
+            for (int i = 0; i < np; ++i) nrm[i] = std::sqrt(std::pow(2.0, 2 * l + 1.5) * std::pow(r.e[i], l + 1.5) / pi15);
+            double s = 0.0;
+            for (int i = 0; i < np; ++i)
+                for (int j = 0; j < np; ++j)
+                    s += r.c[i] * r.c[j] * nrm[i] * nrm[j] * pi15 / (std::pow(2.0, l) * std::pow(r.e[i] + r.e[j], l + 1.5));
+            const double Nc = 1.0 / std::sqrt(s);
+            for (int q = 0; q < 3; ++q) b.center.push_back(xyz[3 * a + q]);
+            b.l.push_back(l);
+            b.nprim.push_back(np);
+            b.atom.push_back(a);
+            for (int i = 0; i < np; ++i) {
+                b.exponent.push_back(r.e[i]);
+                b.coef.push_back(r.c[i] * Nc);
+                b.prim_norm.push_back(nrm[i]);
+            }
+            for (int k = 0; k < ncart(l); ++k) {
+                int x, y, z;
+                cart(l, k, x, y, z);
+                b.fnorm.push_back(1.0 / std::sqrt((double)(dfac(2 * x - 1) * dfac(2 * y - 1) * dfac(2 * z - 1))));
+            }
+        }
+    }
+    b.nb = (int)b.fnorm.size();
+    return PK_OK;
+}
+
+double nuclear_repulsion(const Basis &b) {
+    double E = 0.0;
+    const int n = (int)b.Z.size();
+    for (int i = 0; i < n; ++i)
+        for (int j = i + 1; j < n; ++j) {
+            const double dx = b.xyz[3 * i] - b.xyz[3 * j], dy = b.xyz[3 * i + 1] - b.xyz[3 * j + 1],
+                         dz = b.xyz[3 * i + 2] - b.xyz[3 * j + 2];
+            E += (double)b.Z[i] * b.Z[j] / std::sqrt(dx * dx + dy * dy + dz * dz);
+        }
+    return E;
+}
+
+// S, T, V over shell pairs (row-major nb x nb, symmetric)
+int one_electron(const Basis &b, double *S, double *T, double *V) {
+    const int ns = (int)b.l.size(), nb = b.nb;
+    std::vector<int> f0(ns), p0(ns);
+    for (int s = 0, f = 0, p = 0; s < ns; ++s) {
+        f0[s] = f;
+        p0[s] = p;
+        f += ncart(b.l[s]);
+        p += b.nprim[s];
+    }
+    const int nat = (int)b.Z.size();
+#pragma omp parallel for schedule(dynamic)
+    for (int A = 0; A < ns; ++A) {
+        for (int Bs = 0; Bs <= A; ++Bs) {
+            const int la = b.l[A], lb = b.l[Bs], na = ncart(la), nbb = ncart(lb);
+            const double *Ac = &b.center[3 * A], *Bc = &b.center[3 * Bs];
+            double AB[3], R2 = 0;
+            for (int q = 0; q < 3; ++q) {
+                AB[q] = Ac[q] - Bc[q];
+                R2 += AB[q] * AB[q];
+            }
+            std::vector<double> s(na * nbb, 0.0), t(na * nbb, 0.0), v(na * nbb, 0.0);
+            for (int i = 0; i < b.nprim[A]; ++i)
+                for (int j = 0; j < b.nprim[Bs]; ++j) {
+                    const double al = b.exponent[p0[A] + i], be = b.exponent[p0[Bs] + j], ze = al + be;
+                    const double cc = b.coef[p0[A] + i] * b.prim_norm[p0[A] + i] * b.coef[p0[Bs] + j] * b.prim_norm[p0[Bs] + j];
+                    const double E = std::exp(-al * be / ze * R2) * cc;
+                    double P[3], PA[3], PB[3];
+                    for (int q = 0; q < 3; ++q) {
+                        P[q] = (al * Ac[q] + be * Bc[q]) / ze;
+                        PA[q] = P[q] - Ac[q];
+                        PB[q] = P[q] - Bc[q];
+                    }
+                    // 1-D overlaps O[q][i][j], i <= la, j <= lb + 2 (unit seed)
+                    double O[3][4][6];
+                    const double h = 0.5 / ze;
+                    for (int q = 0; q < 3; ++q) {
+                        for (int ii = 0; ii <= la; ++ii)
+                            for (int jj = 0; jj <= lb + 2; ++jj) {
+                                double val;
+                                if (ii == 0 && jj == 0)
+                                    val = 1.0;
+                                else if (ii > 0) {
+                                    val = PA[q] * O[q][ii - 1][jj];
+                                    if (ii > 1) val += (ii - 1) * h * O[q][ii - 2][jj];
+                                    if (jj > 0) val += jj * h * O[q][ii - 1][jj - 1];
+                                } else {
+                                    val = PB[q] * O[q][0][jj - 1];
+                                    if (jj > 1) val += (jj - 1) * h * O[q][0][jj - 2];
+                                }
+                                O[q][ii][jj] = val;
+                            }
+                    }
+                    const double sf = std::pow(M_PI / ze, 1.5) * E;
+                    for (int ka = 0; ka < na; ++ka) {
+                        int a[3];
+                        cart(la, ka, a[0], a[1], a[2]);
+                        for (int kb = 0; kb < nbb; ++kb) {
+                            int bq[3];
+                            cart(lb, kb, bq[0], bq[1], bq[2]);
+                            double o[3], k1[3];
+                            for (int q = 0; q < 3; ++q) {
+                                const int x = a[q], y = bq[q];
+                                o[q] = O[q][x][y];
+                                k1[q] = be * (2 * y + 1) * O[q][x][y] - 2.0 * be * be * O[q][x][y + 2];
+                                if (y >= 2) k1[q] -= 0.5 * y * (y - 1) * O[q][x][y - 2];
+                            }
+                            s[ka * nbb + kb] += sf * o[0] * o[1] * o[2];
+                            t[ka * nbb + kb] += sf * (k1[0] * o[1] * o[2] + o[0] * k1[1] * o[2] + o[0] * o[1] * k1[2]);
+                        }
+                    }
+                    // nuclear attraction, Rys rule with n = (la+lb)/2 + 1 roots
+                    const int n = (la + lb) / 2 + 1;
+                    const double vf = 2.0 * M_PI / ze * E;
+                    for (int c = 0; c < nat; ++c) {
+                        const double *C = &b.xyz[3 * c];
+                        double PC[3], T2 = 0;
+                        for (int q = 0; q < 3; ++q) {
+                            PC[q] = P[q] - C[q];
+                            T2 += PC[q] * PC[q];
+                        }
+                        double u[8], w[8];
+                        rys_rule_host(n, ze * T2, u, w);
+                        for (int r = 0; r < n; ++r) {
+                            // 1-D integrals I[q][i][j], i <= la, j <= lb by VRR on i then HRR to j
+                            double I[3][4][4];
+                            for (int q = 0; q < 3; ++q) {
+                                double g[8];
+                                const double c00 = PA[q] - u[r] * PC[q], b10 = (1.0 - u[r]) * h;
+                                g[0] = 1.0;
+                                for (int e = 1; e <= la + lb; ++e) {
+                                    g[e] = c00 * g[e - 1];
+                                    if (e > 1) g[e] += (e - 1) * b10 * g[e - 2];
+                                }
+                                double H2[8][4];
+                                for (int e = 0; e <= la + lb; ++e) H2[e][0] = g[e];
+                                for (int jj = 1; jj <= lb; ++jj)
+                                    for (int e = 0; e <= la + lb - jj; ++e) H2[e][jj] = H2[e + 1][jj - 1] + AB[q] * H2[e][jj - 1];
+                                for (int ii = 0; ii <= la; ++ii)
+                                    for (int jj = 0; jj <= lb; ++jj) I[q][ii][jj] = H2[ii][jj];
+                            }
+                            const double fw = -(double)b.Z[c] * vf * w[r];
+                            for (int ka = 0; ka < na; ++ka) {
+                                int a[3];
+                                cart(la, ka, a[0], a[1], a[2]);
+                                for (int kb = 0; kb < nbb; ++kb) {
+                                    int bq[3];
+                                    cart(lb, kb, bq[0], bq[1], bq[2]);
+                                    v[ka * nbb + kb] += fw * I[0][a[0]][bq[0]] * I[1][a[1]][bq[1]] * I[2][a[2]][bq[2]];
+                                }
+                            }
+                        }
+                    }
+                }
+            for (int ka = 0; ka < na; ++ka)
+                for (int kb = 0; kb < nbb; ++kb) {
+                    const int i = f0[A] + ka, j = f0[Bs] + kb;
+                    const double nn = b.fnorm[i] * b.fnorm[j];
+                    const size_t x = (size_t)i * nb + j, y = (size_t)j * nb + i;
+                    if (S) S[x] = S[y] = nn * s[ka * nbb + kb];
+                    if (T) T[x] = T[y] = nn * t[ka * nbb + kb];
+                    if (V) V[x] = V[y] = nn * v[ka * nbb + kb];
+                }
+        }
+    }
+    return PK_OK;
+}
+
+}  // namespace pkh
+
+// ====================================================================== C ABI
+extern "C" {
+
+struct pk_basis {
+    pkh::Basis b;
+};
+
+const char *pk_host_last_error(void) { return pkh::g_err.c_str(); }
+
+int pk_basis_load(const char *path, int natoms, const int32_t *Z, const double *xyz, pk_basis **out) {
+    if (!path || !Z || !xyz || !out || natoms <= 0) return pkh::fail(PK_ERR_ARG, "pk_basis_load: bad arguments");
+    auto *p = new pk_basis;
+    const int rc = pkh::load_basis(path, natoms, Z, xyz, p->b);
+    if (rc != PK_OK) {
+        delete p;
+        return rc;
+    }
+    for (int l : p->b.l)
+        if (l > 3) {
+            delete p;
+            return pkh::fail(PK_ERR_UNSUPPORTED, "pk_basis_load: shells above f are not supported");
+        }
+    *out = p;
+    return PK_OK;
+}
+
+void pk_basis_free(pk_basis *b) { delete b; }
+
+int pk_basis_nbasis(const pk_basis *b) { return b ? b->b.nb : -1; }
+
+int pk_basis_describe(const pk_basis *b, pk_basis_desc *d) {
+    if (!b || !d) return pkh::fail(PK_ERR_ARG, "pk_basis_describe: null argument");
+    d->nshells = (int32_t)b->b.l.size();
+    d->center = b->b.center.data();
+    d->l = b->b.l.data();
+    d->nprim = b->b.nprim.data();
+    d->exponent = b->b.exponent.data();
+    d->coef = b->b.coef.data();
+    d->prim_norm = b->b.prim_norm.data();
+    return PK_OK;
+}
+
+int pk_one_electron(const pk_basis *b, double *S, double *T, double *V) {
+    if (!b) return pkh::fail(PK_ERR_ARG, "pk_one_electron: null basis");
+    return pkh::one_electron(b->b, S, T, V);
+}
+
+double pk_nuclear_repulsion(const pk_basis *b) { return b ? pkh::nuclear_repulsion(b->b) : 0.0; }
+
+}  // extern "C"
+
+const pkh::Basis &pkh_basis_of(const pk_basis *b) { return b->b; }

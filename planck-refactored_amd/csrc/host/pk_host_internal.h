@@ -1,0 +1,30 @@
+// pk_host_internal.h — host-side basis record shared by pk_host.cpp and pk_scf.hip.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "pk_scf.h"
+
+namespace pkh {
+
+struct Basis {
+    int nb = 0;
+    std::vector<int32_t> Z;
+    std::vector<double> xyz;  // Bohr
+    std::vector<double> center;
+    std::vector<int32_t> l, nprim, atom;
+    std::vector<double> exponent, coef, prim_norm;
+    std::vector<double> fnorm;  // component norm per basis function
+};
+
+extern thread_local std::string g_err;
+int fail(int code, const std::string &m);
+int load_basis(const char *path, int natoms, const int32_t *Z, const double *xyz, Basis &b);
+double nuclear_repulsion(const Basis &b);
+int one_electron(const Basis &b, double *S, double *T, double *V);
+
+}  // namespace pkh
+
+const pkh::Basis &pkh_basis_of(const pk_basis *b);

@@ -38,6 +38,12 @@ class Stats(ctypes.Structure):
                 ("last_eri_kernel_ms", ctypes.c_double), ("kernel_launches", ctypes.c_int64)]
 
 
+class ClassStats(ctypes.Structure):
+    _fields_ = [("la", ctypes.c_int32), ("lb", ctypes.c_int32), ("lc", ctypes.c_int32), ("ld", ctypes.c_int32),
+                ("kernels", ctypes.c_int32), ("quartets", ctypes.c_int64), ("primitive_quartets", ctypes.c_int64),
+                ("model_flops", ctypes.c_double), ("ms_total", ctypes.c_double), ("builds", ctypes.c_int64)]
+
+
 _lib = None
 
 
@@ -60,6 +66,9 @@ def load():
     lib.pk_fock_jk_uhf.argtypes = [vp, vp, vp, vp, vp, vp, ip]
     lib.pk_schwarz.argtypes = [vp, vp, ip]
     lib.pk_eri_tensor.argtypes = [vp, vp]
+    lib.pk_ctx_profile.argtypes = [vp, ip]
+    lib.pk_ctx_class_count.argtypes = [vp]
+    lib.pk_ctx_class_stats.argtypes = [vp, ip, ctypes.POINTER(ClassStats)]
     lib.pk_last_error.restype = ctypes.c_char_p
     _lib = lib
     return lib
@@ -134,6 +143,17 @@ class FockEngine:
         s = Stats()
         _check(load().pk_ctx_stats(self.ctx, ctypes.byref(s)), "pk_ctx_stats")
         return {k: getattr(s, k) for k, _ in Stats._fields_}
+
+    def profile(self, enable=True):
+        _check(load().pk_ctx_profile(self.ctx, int(bool(enable))), "pk_ctx_profile")
+
+    def class_stats(self):
+        out = []
+        for i in range(load().pk_ctx_class_count(self.ctx)):
+            s = ClassStats()
+            _check(load().pk_ctx_class_stats(self.ctx, i, ctypes.byref(s)), "pk_ctx_class_stats")
+            out.append({k: getattr(s, k) for k, _ in ClassStats._fields_})
+        return out
 
     def schwarz(self):
         Q = np.zeros((self.nbasis, self.nbasis))
