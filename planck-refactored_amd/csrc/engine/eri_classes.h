@@ -3,7 +3,7 @@
 // translation unit per shard so the classes compile in parallel.
 #pragma once
 
-#include "eri_rys.h"
+#include "eri_kernels.h"
 
 #define PK_MAX_L 2  // s, p, d shells
 
@@ -21,7 +21,7 @@
 
 namespace pk {
 
-constexpr int kAccBudget = 96;  // contracted accumulators per lane
+constexpr int kAccBudget = 48;  // contracted Rys accumulators per lane
 
 constexpr int chunk_size(int la, int lb, int lc, int ld) {
     const int ne = nlev(la, la + lb), ncd = ncart(lc) * ncart(ld);
@@ -30,26 +30,48 @@ constexpr int chunk_size(int la, int lb, int lc, int ld) {
     return ch < ncd ? ch : ncd;
 }
 
+// classes with an Obara-Saika kernel: L <= 4 and [e0|f0] accumulators <= 54
+constexpr bool os_available(int la, int lb, int lc, int ld) {
+    return la + lb + lc + ld <= 4 && nlev(la, la + lb) * nlev(lc, lc + ld) <= 54;
+}
+
 template <int LA, int LB, int LC, int LD, int C0>
-void launch_jk(const JKArgs &a, int grid, hipStream_t s, int &nlaunch) {
+void launch_jk_rys(const JKArgs &a, int grid, hipStream_t s, int &nlaunch) {
     constexpr int NCD = ncart(LC) * ncart(LD);
     if constexpr (C0 < NCD) {
         constexpr int CH = chunk_size(LA, LB, LC, LD);
         constexpr int C1 = C0 + CH < NCD ? C0 + CH : NCD;
-        hipLaunchKernelGGL((k_jk<LA, LB, LC, LD, C0, C1>), dim3(grid), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((k_jk<LA, LB, LC, LD, ALG_RYS, C0, C1>), dim3(grid), dim3(256), 0, s, a);
         ++nlaunch;
-        launch_jk<LA, LB, LC, LD, C1>(a, grid, s, nlaunch);
+        launch_jk_rys<LA, LB, LC, LD, C1>(a, grid, s, nlaunch);
     }
 }
 
 template <int LA, int LB, int LC, int LD, int C0>
-void launch_eri(const EriArgs &a, int grid, hipStream_t s) {
+void launch_eri_rys(const EriArgs &a, int grid, hipStream_t s) {
     constexpr int NCD = ncart(LC) * ncart(LD);
     if constexpr (C0 < NCD) {
         constexpr int CH = chunk_size(LA, LB, LC, LD);
         constexpr int C1 = C0 + CH < NCD ? C0 + CH : NCD;
-        hipLaunchKernelGGL((k_eri<LA, LB, LC, LD, C0, C1>), dim3(grid), dim3(256), 0, s, a);
-        launch_eri<LA, LB, LC, LD, C1>(a, grid, s);
+        hipLaunchKernelGGL((k_eri<LA, LB, LC, LD, ALG_RYS, C0, C1>), dim3(grid), dim3(256), 0, s, a);
+        launch_eri_rys<LA, LB, LC, LD, C1>(a, grid, s);
+    }
+}
+
+template <int LA, int LB, int LC, int LD>
+void launch_jk_os(const JKArgs &a, int grid, hipStream_t s, int &nlaunch) {
+    if constexpr (os_available(LA, LB, LC, LD)) {
+        constexpr int NCD = ncart(LC) * ncart(LD);
+        hipLaunchKernelGGL((k_jk<LA, LB, LC, LD, ALG_OS, 0, NCD>), dim3(grid), dim3(256), 0, s, a);
+        ++nlaunch;
+    }
+}
+
+template <int LA, int LB, int LC, int LD>
+void launch_eri_os(const EriArgs &a, int grid, hipStream_t s) {
+    if constexpr (os_available(LA, LB, LC, LD)) {
+        constexpr int NCD = ncart(LC) * ncart(LD);
+        hipLaunchKernelGGL((k_eri<LA, LB, LC, LD, ALG_OS, 0, NCD>), dim3(grid), dim3(256), 0, s, a);
     }
 }
 

@@ -4,7 +4,28 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 namespace pk {
+
+// Compile-time loop: f(std::integral_constant<int, i>) for i in [B, E).  Every
+// index built from the loop variable is a constant expression, so the arrays
+// of the kernels stay in registers without relying on loop unrolling.
+template <int B, int E, typename F>
+__host__ __device__ __forceinline__ void sfor(F &&f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        sfor<B + 1, E>(f);
+    }
+}
+// descending: f(i) for i = B, B-1, ..., E+1
+template <int B, int E, typename F>
+__host__ __device__ __forceinline__ void sfor_down(F &&f) {
+    if constexpr (B > E) {
+        f(std::integral_constant<int, B>{});
+        sfor_down<B - 1, E>(f);
+    }
+}
 
 // ---------------------------------------------------------------- device records
 // One shell pair (a, b) with L_a >= L_b ("first" shell carries the larger L).
@@ -15,6 +36,7 @@ struct PairDev {
     int diag;  // 1 when both shells are the same shell
     int pad;
     double ab[3];
+    double qmin;  // smallest Schwarz bound Q_ij over the pair's functions
 };
 
 // One primitive pair: zeta = alpha + beta, p = Gaussian product centre,

@@ -1,0 +1,249 @@
+// eri_kernels.h — the per-class shell-quartet kernels (gfx950, FP64).
+//
+// One lane owns one contracted shell quartet (ab|cd) of a fixed angular class
+// (LA >= LB, LC >= LD, bra class >= ket class).  The primitive loops run ket
+// primitives outer (per lane) and bra primitives inner (the bra pair is the
+// same for the whole wave, so its data comes through scalar loads).  Two
+// algorithms share the digestion:
+//   ALG_OS  : Obara-Saika / Head-Gordon-Pople VRR to the contracted [e0|f0],
+//             then the A->B and C->D transfers once per quartet;
+//   ALG_RYS : Rys quadrature, C->D transfer on the 2-D integrals, contracted
+//             [e0|cd) for a chunk [C0, C1) of ket components, then A->B.
+// J/K digestion: for the unique quartet with degeneracy factor d and value v,
+//   A_ab += 2 d v P_cd, A_cd += 2 d v P_ab           (J = A + A^T)
+//   B_ac += d v P_bd, B_bc += d v P_ad, B_ad += d v P_bc, B_bd += d v P_ac   (K = B + B^T)
+// with the reference's screening of every AO quartet whose Q_ab Q_cd is below
+// tol_eri (src/integrals/os.cpp:1475); A_ab is reduced over the wave first,
+// everything else is added with FP64 atomics.
+#pragma once
+
+#include "eri_prim.h"
+
+namespace pk {
+
+enum { ALG_OS = 0, ALG_RYS = 1 };
+
+struct JKArgs {
+    const int *__restrict__ wave_bra;
+    const int *__restrict__ wave_start;
+    const int *__restrict__ wave_count;
+    const int *__restrict__ kets;
+    const PairDev *__restrict__ pairs;
+    const PrimDev *__restrict__ prims;
+    const double *__restrict__ P;  // density scaled by component norms
+    const double *__restrict__ Q;  // Schwarz per function pair (normalised functions)
+    double *A;                     // J accumulator, one orientation
+    double *B;                     // K accumulator, one orientation
+    double tol;
+    int nb;
+    int nwaves;
+};
+
+struct EriArgs {
+    const int *__restrict__ bra;
+    const int *__restrict__ ket;
+    const PairDev *__restrict__ pairs;
+    const PrimDev *__restrict__ prims;
+    double *out;  // [nquart][NA NB NC ND] raw contracted Cartesian ERIs
+    int nquart;
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Contracted Rys intermediates [e0|cd) of the chunk [C0, C1): acc[NE][NCH].
+template <int LA, int LB, int LC, int LD, int C0, int C1>
+__device__ __forceinline__ void rys_contract(const PairDev &pb, const PairDev &pk, int nq, int np,
+                                             const PrimDev *__restrict__ prims,
+                                             double (&acc)[Cls<LA, LB, LC, LD>::NE][C1 - C0]) {
+    using K = Cls<LA, LB, LC, LD>;
+    sfor<0, K::NE>([&](auto i_) { sfor<0, C1 - C0>([&](auto j_) { acc[PK_CI(i_)][PK_CI(j_)] = 0.0; }); });
+    const double cdv[3] = {pk.ab[0], pk.ab[1], pk.ab[2]};
+    for (int kq = 0; kq < nq; ++kq) {
+        const PrimDev q = prims[pk.prim_off + kq];
+        for (int kp = 0; kp < np; ++kp) {
+            const PrimDev p = prims[pb.prim_off + kp];
+            rys_prim<LA, LB, LC, LD, C0, C1>(p, q, cdv, acc);
+        }
+    }
+}
+
+// Contracted (ab|cd) of all ket components by OS/HGP: V[NAB][NCD].
+template <int LA, int LB, int LC, int LD>
+__device__ __forceinline__ void os_values(const PairDev &pb, const PairDev &pk, int nq, int np,
+                                          const PrimDev *__restrict__ prims,
+                                          double (&V)[Cls<LA, LB, LC, LD>::NAB][Cls<LA, LB, LC, LD>::NCD]) {
+    using K = Cls<LA, LB, LC, LD>;
+    constexpr int NE = K::NE, NF = K::NF, NAB = K::NAB;
+    double acc[NE][NF];
+    sfor<0, NE>([&](auto i_) { sfor<0, NF>([&](auto j_) { acc[PK_CI(i_)][PK_CI(j_)] = 0.0; }); });
+    for (int kq = 0; kq < nq; ++kq) {
+        const PrimDev q = prims[pk.prim_off + kq];
+        for (int kp = 0; kp < np; ++kp) {
+            const PrimDev p = prims[pb.prim_off + kp];
+            os_prim<LA, LB, LC, LD>(p, q, acc);
+        }
+    }
+    // A->B per ket column f, then C->D per bra component ab
+    double Tb[NAB][NF];
+    sfor<0, NF>([&](auto jf_) {
+        constexpr int jf = PK_CI(jf_);
+        double col[NE], out[NAB];
+        sfor<0, NE>([&](auto i_) { col[PK_CI(i_)] = acc[PK_CI(i_)][jf]; });
+        hrr_step<LA, LB, 0>(col, pb.ab, out);
+        sfor<0, NAB>([&](auto i_) { Tb[PK_CI(i_)][jf] = out[PK_CI(i_)]; });
+    });
+    sfor<0, NAB>([&](auto i_) { hrr_step<LC, LD, 0>(Tb[PK_CI(i_)], pk.ab, V[PK_CI(i_)]); });
+}
+
+// J/K kernel: one wave = one bra pair and up to 64 ket pairs.
+template <int LA, int LB, int LC, int LD, int ALG, int C0, int C1>
+__global__ __launch_bounds__(256) void k_jk(JKArgs a) {
+    using K = Cls<LA, LB, LC, LD>;
+    constexpr int NA = K::NA, NB = K::NB, ND = K::ND, NAB = K::NAB, NE = K::NE, NCH = C1 - C0;
+    const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+    if (wave >= a.nwaves) return;
+    const int lane = threadIdx.x & 63;
+    const int bra = a.wave_bra[wave];
+    const int cnt = a.wave_count[wave];
+    const bool active = lane < cnt;
+    const int ket = a.kets[a.wave_start[wave] + (active ? lane : 0)];
+    const PairDev pb = a.pairs[bra];
+    const PairDev pk = a.pairs[ket];
+    const int nq = active ? pk.nprim : 0;
+
+    double deg = 1.0;
+    if (pb.diag) deg *= 0.5;
+    if (pk.diag) deg *= 0.5;
+    if (bra == ket) deg *= 0.5;
+    if (!active) deg = 0.0;
+    // every function quartet passes the Schwarz test when the smallest
+    // function-pair bounds do; otherwise each one is tested
+    const bool all_pass = pb.qmin * pk.qmin >= a.tol;
+    const int nb = a.nb;
+    const double *__restrict__ P = a.P;
+
+    double jab[NAB], kac[NA], kbc[NB], kad[NA][ND], kbd[NB][ND];
+    sfor<0, NAB>([&](auto i_) { jab[PK_CI(i_)] = 0.0; });
+    sfor<0, NA>([&](auto i_) { kac[PK_CI(i_)] = 0.0; sfor<0, ND>([&](auto d_) { kad[PK_CI(i_)][PK_CI(d_)] = 0.0; }); });
+    sfor<0, NB>([&](auto i_) { kbc[PK_CI(i_)] = 0.0; sfor<0, ND>([&](auto d_) { kbd[PK_CI(i_)][PK_CI(d_)] = 0.0; }); });
+
+    // digestion of one ket component (ic, id) with its bra block v[NAB]
+    auto digest = [&](auto jj_, const double (&v)[NAB]) {
+        constexpr int jj = PK_CI(jj_);
+        constexpr int ic = (C0 + jj) / ND, id = (C0 + jj) % ND;
+        const int cf = pk.f0 + ic, df = pk.f1 + id;
+        const double Pcd = P[(size_t)cf * nb + df];
+        const double Qcd = a.Q[(size_t)cf * nb + df];
+        double jcd = 0.0;
+        sfor<0, NA>([&](auto ia_) {
+            constexpr int ia = PK_CI(ia_);
+            const int af = pb.f0 + ia;
+            const double Pac = P[(size_t)af * nb + cf], Pad = P[(size_t)af * nb + df];
+            sfor<0, NB>([&](auto ib_) {
+                constexpr int ib = PK_CI(ib_);
+                const int bf = pb.f1 + ib;
+                const size_t abi = (size_t)af * nb + bf;
+                const double m = (all_pass || a.Q[abi] * Qcd >= a.tol) ? deg : 0.0;
+                const double dv = m * v[ia * NB + ib];
+                jcd = fma(dv, P[abi], jcd);
+                jab[ia * NB + ib] = fma(dv, Pcd, jab[ia * NB + ib]);
+                kac[ia] = fma(dv, P[(size_t)bf * nb + df], kac[ia]);
+                kad[ia][id] = fma(dv, P[(size_t)bf * nb + cf], kad[ia][id]);
+                kbc[ib] = fma(dv, Pad, kbc[ib]);
+                kbd[ib][id] = fma(dv, Pac, kbd[ib][id]);
+            });
+        });
+        if (active) unsafeAtomicAdd(&a.A[(size_t)cf * nb + df], 2.0 * jcd);
+        // last ket component of this c in the chunk: flush the (a|c), (b|c) sums
+        if constexpr (id == ND - 1 || jj == NCH - 1) {
+            if (active) {
+                sfor<0, NA>([&](auto ia_) {
+                    unsafeAtomicAdd(&a.B[(size_t)(pb.f0 + PK_CI(ia_)) * nb + cf], kac[PK_CI(ia_)]);
+                });
+                sfor<0, NB>([&](auto ib_) {
+                    unsafeAtomicAdd(&a.B[(size_t)(pb.f1 + PK_CI(ib_)) * nb + cf], kbc[PK_CI(ib_)]);
+                });
+            }
+            sfor<0, NA>([&](auto i_) { kac[PK_CI(i_)] = 0.0; });
+            sfor<0, NB>([&](auto i_) { kbc[PK_CI(i_)] = 0.0; });
+        }
+    };
+
+    if constexpr (ALG == ALG_OS) {
+        static_assert(C0 == 0 && C1 == K::NCD, "the OS kernels take all ket components");
+        double V[NAB][K::NCD];
+        os_values<LA, LB, LC, LD>(pb, pk, nq, pb.nprim, a.prims, V);
+        sfor<0, NCH>([&](auto jj_) {
+            double v[NAB];
+            sfor<0, NAB>([&](auto i_) { v[PK_CI(i_)] = V[PK_CI(i_)][PK_CI(jj_)]; });
+            digest(jj_, v);
+        });
+    } else {
+        double acc[NE][NCH];
+        rys_contract<LA, LB, LC, LD, C0, C1>(pb, pk, nq, pb.nprim, a.prims, acc);
+        sfor<0, NCH>([&](auto jj_) {
+            double col[NE], v[NAB];
+            sfor<0, NE>([&](auto i_) { col[PK_CI(i_)] = acc[PK_CI(i_)][PK_CI(jj_)]; });
+            hrr_step<LA, LB, 0>(col, pb.ab, v);
+            digest(jj_, v);
+        });
+    }
+    if (active) {
+        sfor<0, ND>([&](auto id_) {
+            constexpr int id = PK_CI(id_);
+            constexpr bool present = [] {
+                for (int j = 0; j < NCH; ++j)
+                    if ((C0 + j) % ND == id) return true;
+                return false;
+            }();
+            if constexpr (present) {
+                const int df = pk.f1 + id;
+                sfor<0, NA>([&](auto ia_) {
+                    unsafeAtomicAdd(&a.B[(size_t)(pb.f0 + PK_CI(ia_)) * nb + df], kad[PK_CI(ia_)][id]);
+                });
+                sfor<0, NB>([&](auto ib_) {
+                    unsafeAtomicAdd(&a.B[(size_t)(pb.f1 + PK_CI(ib_)) * nb + df], kbd[PK_CI(ib_)][id]);
+                });
+            }
+        });
+    }
+    sfor<0, NAB>([&](auto i_) {
+        constexpr int i = PK_CI(i_);
+        const double s = wave_sum(jab[i]);
+        if (lane == 0) unsafeAtomicAdd(&a.A[(size_t)(pb.f0 + i / NB) * nb + pb.f1 + i % NB], 2.0 * s);
+    });
+}
+
+// Raw contracted ERIs of arbitrary quartets of one class (Schwarz table, tests).
+template <int LA, int LB, int LC, int LD, int ALG, int C0, int C1>
+__global__ __launch_bounds__(256) void k_eri(EriArgs a) {
+    using K = Cls<LA, LB, LC, LD>;
+    constexpr int NAB = K::NAB, NCD = K::NCD, NE = K::NE, NCH = C1 - C0;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= a.nquart) return;
+    const PairDev pb = a.pairs[a.bra[q]];
+    const PairDev pk = a.pairs[a.ket[q]];
+    double *o = a.out + (size_t)q * K::NABCD;
+    if constexpr (ALG == ALG_OS) {
+        double V[NAB][NCD];
+        os_values<LA, LB, LC, LD>(pb, pk, pk.nprim, pb.nprim, a.prims, V);
+        sfor<0, NAB>([&](auto i_) {
+            sfor<0, NCD>([&](auto j_) { o[(size_t)PK_CI(i_) * NCD + PK_CI(j_)] = V[PK_CI(i_)][PK_CI(j_)]; });
+        });
+    } else {
+        double acc[NE][NCH];
+        rys_contract<LA, LB, LC, LD, C0, C1>(pb, pk, pk.nprim, pb.nprim, a.prims, acc);
+        sfor<0, NCH>([&](auto jj_) {
+            double col[NE], v[NAB];
+            sfor<0, NE>([&](auto i_) { col[PK_CI(i_)] = acc[PK_CI(i_)][PK_CI(jj_)]; });
+            hrr_step<LA, LB, 0>(col, pb.ab, v);
+            sfor<0, NAB>([&](auto i_) { o[(size_t)PK_CI(i_) * NCD + C0 + PK_CI(jj_)] = v[PK_CI(i_)]; });
+        });
+    }
+}
+
+}  // namespace pk

@@ -1,0 +1,333 @@
+// eri_prim.h — per-primitive-quartet building blocks of the ERI kernels (FP64):
+// the Boys function, the Rys rule, the Obara-Saika / Head-Gordon-Pople vertical
+// recurrence and the Rys 2-D integrals, unrolled per angular class at compile
+// time (sfor), so every intermediate lives in registers.
+//
+// Reference semantics restated (the contracted Cartesian ERIs are the same
+// numbers, computed in another order):
+//   Boys F_m(T) ............ src/lookup/boys.cpp:1592-1618 (here: accurate to ~1e-15 relative)
+//   OS VRR ................. src/integrals/os.cpp:618-828
+//   Rys rule, 2-D VRR ...... src/integrals/rys_roots.cpp:223-235, rys.cpp:250-280, 367-485
+//   A->B / C->D transfer ... src/integrals/os.cpp:837-879, 415-459
+#pragma once
+
+#include "eri_common.h"
+
+#define PK_RYS_COEF_QUALIFIER static __device__ const
+#include "rys_table.inc"
+#define PK_BOYS_QUALIFIER static __device__ const
+#include "boys_table.inc"
+
+#define PK_CI(v) decltype(v)::value
+
+namespace pk {
+
+// ---------------------------------------------------------------- Boys
+// F_0..F_M(T): 8-term Taylor series of F_M about the nearest grid point
+// (step 1/16) and downward recursion below 36; F_0 = sqrt(pi/T)/2 and the
+// upward recursion at and above 36.
+template <int M>
+__device__ __forceinline__ void boys(double T, double (&F)[M + 1]) {
+    static_assert(M + 8 <= PK_BOYS_COLS, "Boys table too narrow for this class");
+    const double eT = exp(-T);
+    if (T < (double)PK_BOYS_XMAX) {
+        const int i = (int)(T * PK_BOYS_STEP_INV + 0.5);
+        const double d = T - (double)i * (1.0 / PK_BOYS_STEP_INV);
+        const double *row = pk_boys_table + i * PK_BOYS_COLS + M;
+        double s = row[7];
+        s = fma(s, -d * (1.0 / 7.0), row[6]);
+        s = fma(s, -d * (1.0 / 6.0), row[5]);
+        s = fma(s, -d * (1.0 / 5.0), row[4]);
+        s = fma(s, -d * (1.0 / 4.0), row[3]);
+        s = fma(s, -d * (1.0 / 3.0), row[2]);
+        s = fma(s, -d * (1.0 / 2.0), row[1]);
+        s = fma(s, -d, row[0]);
+        F[M] = s;
+        const double t2 = 2.0 * T;
+        sfor_down<M, 0>([&](auto m_) {
+            constexpr int m = PK_CI(m_);
+            F[m - 1] = fma(t2, F[m], eT) * (1.0 / (2 * m - 1));
+        });
+    } else {
+        const double it2 = 0.5 / T;
+        F[0] = 0.88622692545275801365 * sqrt(1.0 / T);  // sqrt(pi/T)/2
+        sfor<1, M + 1>([&](auto m_) {
+            constexpr int m = PK_CI(m_);
+            F[m] = ((2 * m - 1) * F[m - 1] - eT) * it2;
+        });
+    }
+}
+
+// ---------------------------------------------------------------- Rys rule
+// Root r of the n-point rule of e^{-T u}/(2 sqrt u) du on [0,1]: Chebyshev fits
+// below TMAX(n), the Gauss-Hermite limit above.
+struct RysCtx {
+    const double *c;  // coefficients of T's interval (null in the large-T limit)
+    double x, it, sit;
+};
+
+template <int N>
+__device__ __forceinline__ RysCtx rys_ctx(double T) {
+    RysCtx k;
+    if (T >= pk_rys_tmax[N]) {
+        k.c = nullptr;
+        k.it = 1.0 / T;
+        k.sit = sqrt(k.it);
+        k.x = 0.0;
+    } else {
+        const int i = (int)T;
+        k.c = pk_rys_coef + pk_rys_base[N] + i * (2 * N * (PK_RYS_DEG + 1));
+        k.x = 2.0 * (T - (double)i) - 1.0;
+        k.it = k.sit = 0.0;
+    }
+    return k;
+}
+
+__device__ __forceinline__ double cheb(const double *cf, double x) {
+    const double x2 = 2.0 * x;
+    double b1 = 0.0, b2 = 0.0;
+#pragma unroll
+    for (int j = PK_RYS_DEG; j >= 1; --j) {
+        const double t = fma(x2, b1, cf[j] - b2);
+        b2 = b1;
+        b1 = t;
+    }
+    return fma(x, b1, cf[0] - b2);
+}
+
+template <int N>
+__device__ __forceinline__ void rys_root(const RysCtx &k, int r, double &u, double &w) {
+    if (k.c == nullptr) {
+        u = pk_rys_ax2[pk_rys_abase[N] + r] * k.it;
+        w = pk_rys_aw[pk_rys_abase[N] + r] * k.sit;
+    } else {
+        u = cheb(k.c + r * (PK_RYS_DEG + 1), k.x);
+        w = cheb(k.c + (N + r) * (PK_RYS_DEG + 1), k.x);
+    }
+}
+
+// ---------------------------------------------------------------- class constants
+template <int LA, int LB, int LC, int LD>
+struct Cls {
+    static constexpr int E = LA + LB, F = LC + LD, L = E + F, NR = L / 2 + 1;
+    static constexpr int NA = ncart(LA), NB = ncart(LB), NC = ncart(LC), ND = ncart(LD);
+    static constexpr int NAB = NA * NB, NCD = NC * ND;
+    static constexpr int NE = nlev(LA, E);  // [e0| for levels LA..LA+LB
+    static constexpr int NF = nlev(LC, F);  // |f0] for levels LC..LC+LD
+    static constexpr int NABCD = NAB * NCD;
+};
+
+// axis used to build component (x,y,z) from a lower one: the first non-zero
+__host__ __device__ constexpr int build_axis(int x, int y, int z) { return x > 0 ? 0 : (y > 0 ? 1 : 2); }
+__host__ __device__ constexpr int comp_q(int l, int k, int q) {
+    return q == 0 ? cart_x(l, k) : (q == 1 ? cart_y(l, k) : cart_z(l, k));
+}
+// flat index (levels from 0) of component k of level l shifted by s along axis q
+__host__ __device__ constexpr int flat0_shift(int l, int k, int q, int s) {
+    return flat(0, cart_x(l, k) + (q == 0 ? s : 0), cart_y(l, k) + (q == 1 ? s : 0), cart_z(l, k) + (q == 2 ? s : 0));
+}
+
+// ---------------------------------------------------------------- OS / HGP VRR
+// Accumulate [e0|f0]^(0) for |e| in [LA, E], |f| in [LC, F] of one primitive
+// quartet into acc[NE][NF].  V[e][f][m] holds [e0|f0]^(m) for all |e| <= E,
+// |f| <= F (flat level-0 indices): bra side first, then the ket side with the
+// bra-ket coupling term (Obara & Saika 1986, Head-Gordon & Pople 1988).
+template <int LA, int LB, int LC, int LD>
+__device__ __forceinline__ void os_prim(const PrimDev &p, const PrimDev &q,
+                                        double (&acc)[Cls<LA, LB, LC, LD>::NE][Cls<LA, LB, LC, LD>::NF]) {
+    using K = Cls<LA, LB, LC, LD>;
+    constexpr int E = K::E, F = K::F, L = K::L;
+    constexpr int NLE = nlev(0, E), NLF = nlev(0, F);
+    const double zeta = p.zeta, eta = q.zeta;
+    const double inv_de = 1.0 / (zeta + eta);
+    const double rho = zeta * eta * inv_de;
+    const double PQ[3] = {p.p[0] - q.p[0], p.p[1] - q.p[1], p.p[2] - q.p[2]};
+    const double T = rho * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
+    const double pref = p.kc * q.kc * 2.0 * sqrt(rho * 0.31830988618379067154);
+    double Fm[L + 1];
+    boys<L>(T, Fm);
+    double V[NLE][NLF][L + 1];
+    sfor<0, L + 1>([&](auto m_) { V[0][0][PK_CI(m_)] = pref * Fm[PK_CI(m_)]; });
+    const double fp = -eta * inv_de, fq = zeta * inv_de;  // W - P = fp PQ, W - Q = fq PQ
+    const double i2z = 0.5 / zeta, roz = rho / zeta, i2e = 0.5 / eta, roe = rho / eta, i2d = 0.5 * inv_de;
+    sfor<1, E + 1>([&](auto le_) {
+        constexpr int le = PK_CI(le_);
+        sfor<0, ncart(le)>([&](auto k_) {
+            constexpr int k = PK_CI(k_);
+            constexpr int ax = build_axis(cart_x(le, k), cart_y(le, k), cart_z(le, k));
+            constexpr int ei = comp_q(le, k, ax);
+            constexpr int e = flat(0, cart_x(le, k), cart_y(le, k), cart_z(le, k));
+            constexpr int e1 = flat0_shift(le, k, ax, -1);
+            const double pa = p.pa[ax], wp = fp * PQ[ax];
+            sfor<0, L - le + 1>([&](auto m_) {
+                constexpr int m = PK_CI(m_);
+                double t = fma(pa, V[e1][0][m], wp * V[e1][0][m + 1]);
+                if constexpr (ei >= 2) {
+                    constexpr int e2 = flat0_shift(le, k, ax, -2);
+                    t = fma((ei - 1) * i2z, fma(-roz, V[e2][0][m + 1], V[e2][0][m]), t);
+                }
+                V[e][0][m] = t;
+            });
+        });
+    });
+    sfor<1, F + 1>([&](auto lf_) {
+        constexpr int lf = PK_CI(lf_);
+        sfor<0, ncart(lf)>([&](auto kf_) {
+            constexpr int kf = PK_CI(kf_);
+            constexpr int ax = build_axis(cart_x(lf, kf), cart_y(lf, kf), cart_z(lf, kf));
+            constexpr int fi = comp_q(lf, kf, ax);
+            constexpr int f = flat(0, cart_x(lf, kf), cart_y(lf, kf), cart_z(lf, kf));
+            constexpr int f1 = flat0_shift(lf, kf, ax, -1);
+            const double qc = q.pa[ax], wq = fq * PQ[ax];
+            sfor<0, E + 1>([&](auto le_) {
+                constexpr int le = PK_CI(le_);
+                sfor<0, ncart(le)>([&](auto ke_) {
+                    constexpr int ke = PK_CI(ke_);
+                    constexpr int e = nlev(0, le - 1) + ke;
+                    constexpr int ei = comp_q(le, ke, ax);
+                    sfor<0, L - le - lf + 1>([&](auto m_) {
+                        constexpr int m = PK_CI(m_);
+                        double t = fma(qc, V[e][f1][m], wq * V[e][f1][m + 1]);
+                        if constexpr (fi >= 2) {
+                            constexpr int f2 = flat0_shift(lf, kf, ax, -2);
+                            t = fma((fi - 1) * i2e, fma(-roe, V[e][f2][m + 1], V[e][f2][m]), t);
+                        }
+                        if constexpr (ei >= 1) {
+                            constexpr int em = flat0_shift(le, ke, ax, -1);
+                            t = fma(ei * i2d, V[em][f1][m + 1], t);
+                        }
+                        V[e][f][m] = t;
+                    });
+                });
+            });
+        });
+    });
+    sfor<0, K::NE>([&](auto ie_) {
+        sfor<0, K::NF>([&](auto jf_) {
+            constexpr int ie = PK_CI(ie_), jf = PK_CI(jf_);
+            acc[ie][jf] += V[nlev(0, LA - 1) + ie][nlev(0, LC - 1) + jf][0];
+        });
+    });
+}
+
+// ---------------------------------------------------------------- Rys 2-D integrals
+// G[e][c][d] of one axis: 2-D recurrence (e <= E, f <= LC+LD) then the C->D
+// transfer on the ket index.
+template <int E, int LC, int LD>
+__device__ __forceinline__ void axis_2d(double seed, double C00, double D00, double B00, double B10, double B01,
+                                        double CDq, double (&G)[E + 1][LC + 1][LD + 1]) {
+    constexpr int F = LC + LD;
+    double I[E + 1][F + 1];
+    I[0][0] = seed;
+    sfor<1, E + 1>([&](auto e_) {
+        constexpr int e = PK_CI(e_);
+        I[e][0] = C00 * I[e - 1][0];
+        if constexpr (e >= 2) I[e][0] = fma((double)(e - 1) * B10, I[e - 2][0], I[e][0]);
+    });
+    sfor<1, F + 1>([&](auto f_) {
+        constexpr int f = PK_CI(f_);
+        I[0][f] = D00 * I[0][f - 1];
+        if constexpr (f >= 2) I[0][f] = fma((double)(f - 1) * B01, I[0][f - 2], I[0][f]);
+        sfor<1, E + 1>([&](auto e_) {
+            constexpr int e = PK_CI(e_);
+            double t = fma(D00, I[e][f - 1], (double)e * B00 * I[e - 1][f - 1]);
+            if constexpr (f >= 2) t = fma((double)(f - 1) * B01, I[e][f - 2], t);
+            I[e][f] = t;
+        });
+    });
+    double H[E + 1][F + 1][LD + 1];
+    sfor<0, E + 1>([&](auto e_) { sfor<0, F + 1>([&](auto c_) { H[PK_CI(e_)][PK_CI(c_)][0] = I[PK_CI(e_)][PK_CI(c_)]; }); });
+    sfor<1, LD + 1>([&](auto d_) {
+        constexpr int d = PK_CI(d_);
+        sfor<0, F - d + 1>([&](auto c_) {
+            constexpr int c = PK_CI(c_);
+            sfor<0, E + 1>([&](auto e_) {
+                constexpr int e = PK_CI(e_);
+                H[e][c][d] = fma(CDq, H[e][c][d - 1], H[e][c + 1][d - 1]);
+            });
+        });
+    });
+    sfor<0, E + 1>([&](auto e_) {
+        sfor<0, LC + 1>([&](auto c_) {
+            sfor<0, LD + 1>([&](auto d_) {
+                G[PK_CI(e_)][PK_CI(c_)][PK_CI(d_)] = H[PK_CI(e_)][PK_CI(c_)][PK_CI(d_)];
+            });
+        });
+    });
+}
+
+// One primitive quartet by Rys quadrature: accumulate [e0|cd) of the ket
+// components [C0, C1) into acc[NE][C1 - C0]; one root per loop trip.
+template <int LA, int LB, int LC, int LD, int C0, int C1>
+__device__ __forceinline__ void rys_prim(const PrimDev &p, const PrimDev &q, const double cd[3],
+                                         double (&acc)[Cls<LA, LB, LC, LD>::NE][C1 - C0]) {
+    using K = Cls<LA, LB, LC, LD>;
+    constexpr int NR = K::NR, E = K::E;
+    const double zeta = p.zeta, eta = q.zeta;
+    const double inv_de = 1.0 / (zeta + eta);
+    const double rho = zeta * eta * inv_de;
+    const double PQx = p.p[0] - q.p[0], PQy = p.p[1] - q.p[1], PQz = p.p[2] - q.p[2];
+    const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
+    const double fp = -eta * inv_de, fq = zeta * inv_de;
+    const double pref = p.kc * q.kc * 2.0 * sqrt(rho * 0.31830988618379067154);
+    const double i2z = 0.5 / zeta, i2e = 0.5 / eta, roz = rho / zeta, roe = rho / eta;
+    const RysCtx rk = rys_ctx<NR>(T);
+#pragma unroll 1
+    for (int r = 0; r < NR; ++r) {
+        double ur, wr;
+        rys_root<NR>(rk, r, ur, wr);
+        const double B00 = 0.5 * inv_de * ur;
+        const double B10 = i2z * (1.0 - roz * ur);
+        const double B01 = i2e * (1.0 - roe * ur);
+        const double uwp = ur * fp, uwq = ur * fq;
+        double Gx[E + 1][LC + 1][LD + 1], Gy[E + 1][LC + 1][LD + 1], Gz[E + 1][LC + 1][LD + 1];
+        axis_2d<E, LC, LD>(pref * wr, fma(uwp, PQx, p.pa[0]), fma(uwq, PQx, q.pa[0]), B00, B10, B01, cd[0], Gx);
+        axis_2d<E, LC, LD>(1.0, fma(uwp, PQy, p.pa[1]), fma(uwq, PQy, q.pa[1]), B00, B10, B01, cd[1], Gy);
+        axis_2d<E, LC, LD>(1.0, fma(uwp, PQz, p.pa[2]), fma(uwq, PQz, q.pa[2]), B00, B10, B01, cd[2], Gz);
+        sfor<0, K::NE>([&](auto ie_) {
+            constexpr int ie = PK_CI(ie_);
+            constexpr int le = flat_level(LA, ie), ke = flat_k(LA, ie);
+            constexpr int ex = cart_x(le, ke), ey = cart_y(le, ke), ez = cart_z(le, ke);
+            sfor<0, C1 - C0>([&](auto j_) {
+                constexpr int j = PK_CI(j_);
+                constexpr int ic = (C0 + j) / K::ND, id = (C0 + j) % K::ND;
+                constexpr int cx = cart_x(LC, ic), cy = cart_y(LC, ic), cz = cart_z(LC, ic);
+                constexpr int dx = cart_x(LD, id), dy = cart_y(LD, id), dz = cart_z(LD, id);
+                acc[ie][j] = fma(Gx[ex][cx][dx] * Gy[ey][cy][dy], Gz[ez][cz][dz], acc[ie][j]);
+            });
+        });
+    }
+}
+
+// ---------------------------------------------------------------- transfers
+// A -> B transfer: cur over (a levels LA..LA+LB-K) x (b level K) -> out (a, b)
+// of levels (LA, LB).  Used for the bra (A -> B) and, with the ket's angular
+// momenta, for the ket (C -> D).
+template <int LA, int LB, int K>
+__device__ __forceinline__ void hrr_step(const double (&cur)[nlev(LA, LA + LB - K) * ncart(K)], const double ab[3],
+                                         double (&out)[ncart(LA) * ncart(LB)]) {
+    if constexpr (K == LB) {
+        sfor<0, ncart(LA) * ncart(LB)>([&](auto i_) { out[PK_CI(i_)] = cur[PK_CI(i_)]; });
+    } else {
+        constexpr int NK = ncart(K), NK1 = ncart(K + 1);
+        constexpr int NAL = nlev(LA, LA + LB - K - 1);
+        double nxt[NAL * NK1];
+        sfor<0, NK1>([&](auto bp_) {
+            constexpr int bp = PK_CI(bp_);
+            constexpr int bx = cart_x(K + 1, bp), by = cart_y(K + 1, bp), bz = cart_z(K + 1, bp);
+            constexpr int ax = build_axis(bx, by, bz);
+            constexpr int b = cart_index(bx - (ax == 0), by - (ax == 1), bz - (ax == 2));
+            const double abq = ab[ax];
+            sfor<0, NAL>([&](auto ia_) {
+                constexpr int ia = PK_CI(ia_);
+                constexpr int la = flat_level(LA, ia), ka = flat_k(LA, ia);
+                constexpr int iap = flat(LA, cart_x(la, ka) + (ax == 0), cart_y(la, ka) + (ax == 1),
+                                         cart_z(la, ka) + (ax == 2));
+                nxt[ia * NK1 + bp] = fma(abq, cur[ia * NK + b], cur[iap * NK + b]);
+            });
+        });
+        hrr_step<LA, LB, K + 1>(nxt, ab, out);
+    }
+}
+
+}  // namespace pk

@@ -22,8 +22,10 @@
 
 // per-class launchers, defined in pk_kernels.hip
 #define PK_DECL(a, b, c, d, shard)                                                             \
-    void pk_launch_jk_##a##b##c##d(const pk::JKArgs &, int, hipStream_t, int &);               \
-    void pk_launch_eri_##a##b##c##d(const pk::EriArgs &, int, hipStream_t);
+    void pk_launch_jk_rys_##a##b##c##d(const pk::JKArgs &, int, hipStream_t, int &);           \
+    void pk_launch_eri_rys_##a##b##c##d(const pk::EriArgs &, int, hipStream_t);                \
+    void pk_launch_jk_os_##a##b##c##d(const pk::JKArgs &, int, hipStream_t, int &);            \
+    void pk_launch_eri_os_##a##b##c##d(const pk::EriArgs &, int, hipStream_t);
 PK_FOR_EACH_CLASS(PK_DECL)
 #undef PK_DECL
 
@@ -49,11 +51,16 @@ constexpr int kQuartClasses = kPairClasses * (kPairClasses + 1) / 2;
 
 struct ClassFns {
     int la, lb, lc, ld;
-    void (*jk)(const pk::JKArgs &, int, hipStream_t, int &);
-    void (*eri)(const pk::EriArgs &, int, hipStream_t);
+    bool has_os;
+    void (*jk_os)(const pk::JKArgs &, int, hipStream_t, int &);
+    void (*eri_os)(const pk::EriArgs &, int, hipStream_t);
+    void (*jk_rys)(const pk::JKArgs &, int, hipStream_t, int &);
+    void (*eri_rys)(const pk::EriArgs &, int, hipStream_t);
 };
 
-#define PK_ENTRY(a, b, c, d, shard) ClassFns{a, b, c, d, &pk_launch_jk_##a##b##c##d, &pk_launch_eri_##a##b##c##d},
+#define PK_ENTRY(a, b, c, d, shard)                                                                  \
+    ClassFns{a, b, c, d, pk::os_available(a, b, c, d), &pk_launch_jk_os_##a##b##c##d,                \
+             &pk_launch_eri_os_##a##b##c##d, &pk_launch_jk_rys_##a##b##c##d, &pk_launch_eri_rys_##a##b##c##d},
 // Quartet classes (bra class >= ket class), indexed cb (cb + 1) / 2 + ck with
 // pair classes (0,0) (1,0) (1,1) (2,0) (2,1) (2,2).
 const ClassFns kClasses[kQuartClasses] = {PK_FOR_EACH_CLASS(PK_ENTRY)};
@@ -150,6 +157,9 @@ struct pk_ctx {
 
 namespace {
 
+// OS kernels where they exist unless the Rys engine is forced
+bool use_os(const pk_ctx *c, const ClassFns &f) { return f.has_os && c->engine != PK_ENGINE_RYS; }
+
 void free_work(pk_ctx *c) {
     for (ClassWork &w : c->work) {
         if (w.d_wave) (void)hipFree(w.d_wave);
@@ -177,7 +187,7 @@ int run_eri_class(pk_ctx *c, int cb, int ck, const std::vector<int> &bra, const 
     if ((rc = dev_upload(&d_k, ket, c->stream)) != PK_OK) return rc;
     PK_HIP(hipMalloc((void **)&d_o, out.size() * sizeof(double)));
     pk::EriArgs a{d_b, d_k, c->d_pairs, c->d_prims, d_o, nq};
-    f.eri(a, (nq + 255) / 256, c->stream);
+    (use_os(c, f) ? f.eri_os : f.eri_rys)(a, (nq + 255) / 256, c->stream);
     PK_HIP(hipGetLastError());
     PK_HIP(hipMemcpyAsync(out.data(), d_o, out.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     PK_HIP(hipStreamSynchronize(c->stream));
@@ -202,7 +212,7 @@ int build_schwarz(pk_ctx *c) {
         const int na = pk::ncart(la), nbb = pk::ncart(lb), nab = na * nbb;
         for (size_t q = 0; q < idx.size(); ++q) {
             const pk::PairDev &p = c->pairs[idx[q]];
-            double qmax = 0.0;
+            double qmax = 0.0, qmin = 1e300;
             for (int ia = 0; ia < na; ++ia)
                 for (int ib = 0; ib < nbb; ++ib) {
                     const int i = p.f0 + ia, j = p.f1 + ib;
@@ -210,8 +220,10 @@ int build_schwarz(pk_ctx *c) {
                     const double v = c->fnorm[i] * c->fnorm[j] * std::sqrt(std::fabs(raw));
                     c->Q[(size_t)i * nb + j] = c->Q[(size_t)j * nb + i] = v;
                     qmax = std::max(qmax, v);
+                    qmin = std::min(qmin, v);
                 }
             c->pair_q[idx[q]] = qmax;
+            c->pairs[idx[q]].qmin = qmin;
         }
     }
     return PK_OK;
@@ -328,7 +340,8 @@ int build_device(pk_ctx *c, const double *dP, double *dJ, double *dK, double *dG
             PK_HIP(hipEventRecord(w.e0, s));
         }
         const int before = nl;
-        kClasses[w.cls].jk(a, (w.nwaves + 3) / 4, s, nl);
+        const ClassFns &f = kClasses[w.cls];
+        (use_os(c, f) ? f.jk_os : f.jk_rys)(a, (w.nwaves + 3) / 4, s, nl);
         w.kernels = nl - before;
         if (c->profile) PK_HIP(hipEventRecord(w.e1, s));
     }
@@ -461,7 +474,10 @@ int pk_ctx_create(const pk_basis_desc *d, pk_ctx **out) {
     if ((rc = dev_upload(&c->d_pairs, c->pairs, c->stream)) != PK_OK ||
         (rc = dev_upload(&c->d_prims, c->prims, c->stream)) != PK_OK ||
         (rc = dev_upload(&c->d_fnorm, c->fnorm, c->stream)) != PK_OK || (rc = build_schwarz(c)) != PK_OK ||
-        (rc = dev_upload(&c->d_Q, c->Q, c->stream)) != PK_OK) {
+        (rc = dev_upload(&c->d_Q, c->Q, c->stream)) != PK_OK ||
+        hipMemcpyAsync(c->d_pairs, c->pairs.data(), c->pairs.size() * sizeof(pk::PairDev), hipMemcpyHostToDevice,
+                       c->stream) != hipSuccess) {
+        if (rc == PK_OK) rc = fail(PK_ERR_HIP, "pk_ctx_create: pair upload failed");
         const std::string msg = g_err;
         pk_ctx_destroy(c);
         return fail(rc, msg);

@@ -6,11 +6,17 @@
     PK_DEFINE_IF_##shard(a, b, c, d)
 
 #define PK_DEFINE_NOW(a, b, c, d)                                                              \
-    void pk_launch_jk_##a##b##c##d(const pk::JKArgs &x, int g, hipStream_t s, int &n) {       \
-        pk::launch_jk<a, b, c, d, 0>(x, g, s, n);                                              \
+    void pk_launch_jk_rys_##a##b##c##d(const pk::JKArgs &x, int g, hipStream_t s, int &n) {   \
+        pk::launch_jk_rys<a, b, c, d, 0>(x, g, s, n);                                          \
     }                                                                                          \
-    void pk_launch_eri_##a##b##c##d(const pk::EriArgs &x, int g, hipStream_t s) {              \
-        pk::launch_eri<a, b, c, d, 0>(x, g, s);                                                \
+    void pk_launch_eri_rys_##a##b##c##d(const pk::EriArgs &x, int g, hipStream_t s) {          \
+        pk::launch_eri_rys<a, b, c, d, 0>(x, g, s);                                            \
+    }                                                                                          \
+    void pk_launch_jk_os_##a##b##c##d(const pk::JKArgs &x, int g, hipStream_t s, int &n) {    \
+        pk::launch_jk_os<a, b, c, d>(x, g, s, n);                                              \
+    }                                                                                          \
+    void pk_launch_eri_os_##a##b##c##d(const pk::EriArgs &x, int g, hipStream_t s) {           \
+        pk::launch_eri_os<a, b, c, d>(x, g, s);                                                \
     }
 
 #define PK_SKIP(a, b, c, d)

@@ -34,10 +34,12 @@ def _density(n, seed):
     return A + A.T
 
 
+@pytest.mark.parametrize("engine", ["os", "rys"])
 @pytest.mark.parametrize("basis,mol", [("sto-3g", "water"), ("6-31g*", "water"), ("cc-pvdz", "ethylene")])
-def test_schwarz_and_eri_tensor(basis, mol):
+def test_schwarz_and_eri_tensor(basis, mol, engine):
     Z, X = water() if mol == "water" else ETHYLENE
     ref, eng = _pair(basis, Z, X, tol=1e-12)
+    eng.set(engine, 1e-12)
     dq = np.abs(eng.schwarz() - ref.matrix("schwarz")).max()
     print("%s/%s nb=%d  max|dQ| = %.3e" % (mol, basis, ref.nbasis, dq))
     assert dq < ERI_TOL
@@ -46,9 +48,10 @@ def test_schwarz_and_eri_tensor(basis, mol):
     assert d < ERI_TOL
 
 
+@pytest.mark.parametrize("engine", ["os", "rys", "auto"])
 @pytest.mark.parametrize("basis,mol,tol", [("6-31g*", "water", 1e-12), ("cc-pvdz", "ethylene", 1e-10),
                                            ("cc-pvdz", "propane", 1e-12)])
-def test_jk_matches_oracle(basis, mol, tol):
+def test_jk_matches_oracle(basis, mol, tol, engine):
     if mol == "water":
         Z, X = water()
     elif mol == "ethylene":
@@ -56,6 +59,7 @@ def test_jk_matches_oracle(basis, mol, tol):
     else:
         Z, X = alkane(3)
     ref, eng = _pair(basis, Z, X, tol=tol)
+    eng.set(engine, tol)
     P = _density(ref.nbasis, 7)
     J, K = eng.jk(P)
     Jr, Kr = ref.jk(P)
