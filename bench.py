@@ -113,7 +113,7 @@ def main():
     achieved = dom["model_flops"] / (dom["ms_avg"] * 1e-3) * 1e-12
     print("bench: %d steps %.2f ms/step; model %.3e flop/build = %.2f TFLOP/s whole build" % (
         args.steps, ms, st["model_flops"], st["model_flops"] / ms * 1e-9), file=sys.stderr, flush=True)
-    for c in sorted(cls, key=lambda c: -c["ms_avg"])[:8]:
+    for c in sorted(cls, key=lambda c: -c["ms_avg"]):
         print("  class (%d%d|%d%d) x%d kernels: %8.3f ms  %.3e quartets  %.2f TFLOP/s" % (
             c["la"], c["lb"], c["lc"], c["ld"], c["kernels"], c["ms_avg"], c["quartets"],
             c["model_flops"] / (c["ms_avg"] * 1e-3) * 1e-12), file=sys.stderr, flush=True)

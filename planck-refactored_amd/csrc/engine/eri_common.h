@@ -40,8 +40,9 @@ struct PairDev {
 };
 
 // One primitive pair: zeta = alpha + beta, p = Gaussian product centre,
-// pa = P - A (A = first shell), kc = (pi/zeta)^1.5 exp(-alpha beta/zeta |AB|^2)
-// times c_a N_a c_b N_b (contraction coefficient x Nc, primitive norm).
+// pa = P - A (A = first shell), kc = K_ab sqrt(zeta) (4/pi)^(1/4) with
+// K_ab = (pi/zeta)^1.5 exp(-alpha beta/zeta |AB|^2) c_a N_a c_b N_b
+// (contraction coefficient x Nc, primitive norm).
 struct PrimDev {
     double zeta, kc;
     double p[3];

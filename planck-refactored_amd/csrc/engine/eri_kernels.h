@@ -39,15 +39,6 @@ struct JKArgs {
     int nwaves;
 };
 
-struct EriArgs {
-    const int *__restrict__ bra;
-    const int *__restrict__ ket;
-    const PairDev *__restrict__ pairs;
-    const PrimDev *__restrict__ prims;
-    double *out;  // [nquart][NA NB NC ND] raw contracted Cartesian ERIs
-    int nquart;
-};
-
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -216,34 +207,6 @@ __global__ __launch_bounds__(256) void k_jk(JKArgs a) {
         const double s = wave_sum(jab[i]);
         if (lane == 0) unsafeAtomicAdd(&a.A[(size_t)(pb.f0 + i / NB) * nb + pb.f1 + i % NB], 2.0 * s);
     });
-}
-
-// Raw contracted ERIs of arbitrary quartets of one class (Schwarz table, tests).
-template <int LA, int LB, int LC, int LD, int ALG, int C0, int C1>
-__global__ __launch_bounds__(256) void k_eri(EriArgs a) {
-    using K = Cls<LA, LB, LC, LD>;
-    constexpr int NAB = K::NAB, NCD = K::NCD, NE = K::NE, NCH = C1 - C0;
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= a.nquart) return;
-    const PairDev pb = a.pairs[a.bra[q]];
-    const PairDev pk = a.pairs[a.ket[q]];
-    double *o = a.out + (size_t)q * K::NABCD;
-    if constexpr (ALG == ALG_OS) {
-        double V[NAB][NCD];
-        os_values<LA, LB, LC, LD>(pb, pk, pk.nprim, pb.nprim, a.prims, V);
-        sfor<0, NAB>([&](auto i_) {
-            sfor<0, NCD>([&](auto j_) { o[(size_t)PK_CI(i_) * NCD + PK_CI(j_)] = V[PK_CI(i_)][PK_CI(j_)]; });
-        });
-    } else {
-        double acc[NE][NCH];
-        rys_contract<LA, LB, LC, LD, C0, C1>(pb, pk, pk.nprim, pb.nprim, a.prims, acc);
-        sfor<0, NCH>([&](auto jj_) {
-            double col[NE], v[NAB];
-            sfor<0, NE>([&](auto i_) { col[PK_CI(i_)] = acc[PK_CI(i_)][PK_CI(jj_)]; });
-            hrr_step<LA, LB, 0>(col, pb.ab, v);
-            sfor<0, NAB>([&](auto i_) { o[(size_t)PK_CI(i_) * NCD + C0 + PK_CI(jj_)] = v[PK_CI(i_)]; });
-        });
-    }
 }
 
 }  // namespace pk

@@ -18,21 +18,33 @@
 #define PK_BOYS_QUALIFIER static __device__ const
 #include "boys_table.inc"
 
+// PrimDev::kc carries K_ab sqrt(zeta) (4/pi)^(1/4), so that the product of a bra
+// and a ket record times 1/sqrt(zeta+eta) is K_ab K_cd 2 sqrt(rho/pi).
+
 #define PK_CI(v) decltype(v)::value
 
 namespace pk {
 
+// ---------------------------------------------------------------- helpers
+// 1/sqrt(x) to full FP64 precision: hardware estimate and two Newton steps
+__device__ __forceinline__ double rsqrt_nr(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    const double h = 0.5 * x;
+    y = fma(y, fma(-h * y, y, 0.5), y);
+    y = fma(y, fma(-h * y, y, 0.5), y);
+    return y;
+}
+
 // ---------------------------------------------------------------- Boys
 // F_0..F_M(T): 8-term Taylor series of F_M about the nearest grid point
-// (step 1/16) and downward recursion below 36; F_0 = sqrt(pi/T)/2 and the
-// upward recursion at and above 36.
+// (step 1/16) and downward recursion below 36, with e^{-T} = e^{-x_i} e^{-d};
+// F_0 = sqrt(pi/T)/2 and the upward recursion at and above 36.
 template <int M>
 __device__ __forceinline__ void boys(double T, double (&F)[M + 1]) {
     static_assert(M + 8 <= PK_BOYS_COLS, "Boys table too narrow for this class");
-    const double eT = exp(-T);
     if (T < (double)PK_BOYS_XMAX) {
-        const int i = (int)(T * PK_BOYS_STEP_INV + 0.5);
-        const double d = T - (double)i * (1.0 / PK_BOYS_STEP_INV);
+        const int i = (int)fma(T, (double)PK_BOYS_STEP_INV, 0.5);
+        const double d = fma((double)i, -1.0 / PK_BOYS_STEP_INV, T);
         const double *row = pk_boys_table + i * PK_BOYS_COLS + M;
         double s = row[7];
         s = fma(s, -d * (1.0 / 7.0), row[6]);
@@ -43,18 +55,33 @@ __device__ __forceinline__ void boys(double T, double (&F)[M + 1]) {
         s = fma(s, -d * (1.0 / 2.0), row[1]);
         s = fma(s, -d, row[0]);
         F[M] = s;
-        const double t2 = 2.0 * T;
-        sfor_down<M, 0>([&](auto m_) {
-            constexpr int m = PK_CI(m_);
-            F[m - 1] = fma(t2, F[m], eT) * (1.0 / (2 * m - 1));
-        });
+        if constexpr (M > 0) {
+            // e^{-d}, |d| <= 1/32: 8 Taylor terms (next term < 1e-17)
+            double e = fma(-d, 1.0 / 40320.0, 1.0 / 5040.0);
+            e = fma(e, -d, 1.0 / 720.0);
+            e = fma(e, -d, 1.0 / 120.0);
+            e = fma(e, -d, 1.0 / 24.0);
+            e = fma(e, -d, 1.0 / 6.0);
+            e = fma(e, -d, 0.5);
+            e = fma(e, -d, 1.0);
+            e = fma(e, -d, 1.0);
+            const double eT = pk_boys_exp[i] * e;
+            const double t2 = 2.0 * T;
+            sfor_down<M, 0>([&](auto m_) {
+                constexpr int m = PK_CI(m_);
+                F[m - 1] = fma(t2, F[m], eT) * (1.0 / (2 * m - 1));
+            });
+        }
     } else {
-        const double it2 = 0.5 / T;
-        F[0] = 0.88622692545275801365 * sqrt(1.0 / T);  // sqrt(pi/T)/2
-        sfor<1, M + 1>([&](auto m_) {
-            constexpr int m = PK_CI(m_);
-            F[m] = ((2 * m - 1) * F[m - 1] - eT) * it2;
-        });
+        const double r = rsqrt_nr(T);
+        F[0] = 0.88622692545275801365 * r;  // sqrt(pi/T)/2
+        if constexpr (M > 0) {
+            const double eT = exp(-T), it2 = 0.5 * r * r;
+            sfor<1, M + 1>([&](auto m_) {
+                constexpr int m = PK_CI(m_);
+                F[m] = ((2 * m - 1) * F[m - 1] - eT) * it2;
+            });
+        }
     }
 }
 
@@ -71,8 +98,8 @@ __device__ __forceinline__ RysCtx rys_ctx(double T) {
     RysCtx k;
     if (T >= pk_rys_tmax[N]) {
         k.c = nullptr;
-        k.it = 1.0 / T;
-        k.sit = sqrt(k.it);
+        k.sit = rsqrt_nr(T);
+        k.it = k.sit * k.sit;
         k.x = 0.0;
     } else {
         const int i = (int)T;
@@ -139,11 +166,12 @@ __device__ __forceinline__ void os_prim(const PrimDev &p, const PrimDev &q,
     constexpr int E = K::E, F = K::F, L = K::L;
     constexpr int NLE = nlev(0, E), NLF = nlev(0, F);
     const double zeta = p.zeta, eta = q.zeta;
-    const double inv_de = 1.0 / (zeta + eta);
+    const double rs = rsqrt_nr(zeta + eta);
+    const double inv_de = rs * rs;
     const double rho = zeta * eta * inv_de;
     const double PQ[3] = {p.p[0] - q.p[0], p.p[1] - q.p[1], p.p[2] - q.p[2]};
-    const double T = rho * (PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2]);
-    const double pref = p.kc * q.kc * 2.0 * sqrt(rho * 0.31830988618379067154);
+    const double T = rho * fma(PQ[0], PQ[0], fma(PQ[1], PQ[1], PQ[2] * PQ[2]));
+    const double pref = p.kc * q.kc * rs;
     double Fm[L + 1];
     boys<L>(T, Fm);
     double V[NLE][NLF][L + 1];
@@ -264,12 +292,13 @@ __device__ __forceinline__ void rys_prim(const PrimDev &p, const PrimDev &q, con
     using K = Cls<LA, LB, LC, LD>;
     constexpr int NR = K::NR, E = K::E;
     const double zeta = p.zeta, eta = q.zeta;
-    const double inv_de = 1.0 / (zeta + eta);
+    const double rs = rsqrt_nr(zeta + eta);
+    const double inv_de = rs * rs;
     const double rho = zeta * eta * inv_de;
     const double PQx = p.p[0] - q.p[0], PQy = p.p[1] - q.p[1], PQz = p.p[2] - q.p[2];
-    const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
+    const double T = rho * fma(PQx, PQx, fma(PQy, PQy, PQz * PQz));
     const double fp = -eta * inv_de, fq = zeta * inv_de;
-    const double pref = p.kc * q.kc * 2.0 * sqrt(rho * 0.31830988618379067154);
+    const double pref = p.kc * q.kc * rs;
     const double i2z = 0.5 / zeta, i2e = 0.5 / eta, roz = rho / zeta, roe = rho / eta;
     const RysCtx rk = rys_ctx<NR>(T);
 #pragma unroll 1

@@ -1,7 +1,9 @@
 // pk_fock.hip — host side of the MI355X Fock-build engine and its C ABI
-// (include/pk_fock.h).  Kernels live in eri_rys.h; this file builds the shell
-// pairs, the Schwarz table and the screened quartet work lists, and launches
-// the per-class kernels on the context's stream.
+// (include/pk_fock.h).  Per-lane class kernels live in eri_kernels.h (instantiated
+// in pk_kernels.hip), the cooperative kernel for large f classes and for ERI
+// output in eri_generic.h; this file builds the shell pairs, the Schwarz table
+// and the screened quartet work lists, and launches the kernels on the
+// context's stream.
 //
 // Reference path replaced: src/integrals/base.h:57-93 (_compute_2e_fock[_uhf]),
 // src/integrals/os.cpp:1435-1625 and rys.cpp:701-824 (the Fock builds),
@@ -19,13 +21,12 @@
 #include <vector>
 
 #include "eri_classes.h"
+#include "eri_generic.h"
 
 // per-class launchers, defined in pk_kernels.hip
 #define PK_DECL(a, b, c, d, shard)                                                             \
     void pk_launch_jk_rys_##a##b##c##d(const pk::JKArgs &, int, hipStream_t, int &);           \
-    void pk_launch_eri_rys_##a##b##c##d(const pk::EriArgs &, int, hipStream_t);                \
-    void pk_launch_jk_os_##a##b##c##d(const pk::JKArgs &, int, hipStream_t, int &);            \
-    void pk_launch_eri_os_##a##b##c##d(const pk::EriArgs &, int, hipStream_t);
+    void pk_launch_jk_os_##a##b##c##d(const pk::JKArgs &, int, hipStream_t, int &);
 PK_FOR_EACH_CLASS(PK_DECL)
 #undef PK_DECL
 
@@ -51,18 +52,17 @@ constexpr int kQuartClasses = kPairClasses * (kPairClasses + 1) / 2;
 
 struct ClassFns {
     int la, lb, lc, ld;
-    bool has_os;
+    bool has_os;    // Obara-Saika per-lane kernel
+    bool has_tmpl;  // Rys per-lane kernel (else the cooperative kernel)
     void (*jk_os)(const pk::JKArgs &, int, hipStream_t, int &);
-    void (*eri_os)(const pk::EriArgs &, int, hipStream_t);
     void (*jk_rys)(const pk::JKArgs &, int, hipStream_t, int &);
-    void (*eri_rys)(const pk::EriArgs &, int, hipStream_t);
 };
 
-#define PK_ENTRY(a, b, c, d, shard)                                                                  \
-    ClassFns{a, b, c, d, pk::os_available(a, b, c, d), &pk_launch_jk_os_##a##b##c##d,                \
-             &pk_launch_eri_os_##a##b##c##d, &pk_launch_jk_rys_##a##b##c##d, &pk_launch_eri_rys_##a##b##c##d},
+#define PK_ENTRY(a, b, c, d, shard)                                                             \
+    ClassFns{a, b, c, d, pk::os_available(a, b, c, d), pk::rys_tmpl_available(a, b, c, d),      \
+             &pk_launch_jk_os_##a##b##c##d, &pk_launch_jk_rys_##a##b##c##d},
 // Quartet classes (bra class >= ket class), indexed cb (cb + 1) / 2 + ck with
-// pair classes (0,0) (1,0) (1,1) (2,0) (2,1) (2,2).
+// pair classes (0,0) (1,0) (1,1) (2,0) (2,1) (2,2) (3,0) (3,1) (3,2) (3,3).
 const ClassFns kClasses[kQuartClasses] = {PK_FOR_EACH_CLASS(PK_ENTRY)};
 #undef PK_ENTRY
 
@@ -173,6 +173,20 @@ void free_work(pk_ctx *c) {
 
 // Raw contracted ERIs (no component norms) of the quartets (bra[i] | ket[i]),
 // all of class (cb, ck) with cb >= ck.  out gets NA NB NC ND values per quartet.
+// One 64-lane workgroup per item; the dynamic LDS holds the transferred 1-D
+// factors and the quartet's accumulators (at most 123 KB for (ff|ff)).
+int launch_generic(const pk::GenArgs &a, hipStream_t s) {
+    if (a.nitems <= 0) return PK_OK;
+    static const size_t max_bytes = pk::generic_lds_bytes(kMaxL, kMaxL, kMaxL, kMaxL);
+    static const hipError_t attr =
+        hipFuncSetAttribute((const void *)pk::k_generic, hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_bytes);
+    PK_HIP(attr);
+    const size_t bytes = pk::generic_lds_bytes(a.la, a.lb, a.lc, a.ld);
+    hipLaunchKernelGGL(pk::k_generic, dim3(a.nitems), dim3(64), bytes, s, a);
+    PK_HIP(hipGetLastError());
+    return PK_OK;
+}
+
 int run_eri_class(pk_ctx *c, int cb, int ck, const std::vector<int> &bra, const std::vector<int> &ket,
                   std::vector<double> &out) {
     const ClassFns &f = kClasses[quart_class(cb, ck)];
@@ -186,9 +200,19 @@ int run_eri_class(pk_ctx *c, int cb, int ck, const std::vector<int> &bra, const 
     if ((rc = dev_upload(&d_b, bra, c->stream)) != PK_OK) return rc;
     if ((rc = dev_upload(&d_k, ket, c->stream)) != PK_OK) return rc;
     PK_HIP(hipMalloc((void **)&d_o, out.size() * sizeof(double)));
-    pk::EriArgs a{d_b, d_k, c->d_pairs, c->d_prims, d_o, nq};
-    (use_os(c, f) ? f.eri_os : f.eri_rys)(a, (nq + 255) / 256, c->stream);
-    PK_HIP(hipGetLastError());
+    pk::GenArgs a{};
+    a.qbra = d_b;
+    a.qket = d_k;
+    a.pairs = c->d_pairs;
+    a.prims = c->d_prims;
+    a.out = d_o;
+    a.nitems = nq;
+    a.la = f.la;
+    a.lb = f.lb;
+    a.lc = f.lc;
+    a.ld = f.ld;
+    a.eri_mode = 1;
+    if ((rc = launch_generic(a, c->stream)) != PK_OK) return rc;
     PK_HIP(hipMemcpyAsync(out.data(), d_o, out.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     PK_HIP(hipStreamSynchronize(c->stream));
     (void)hipFree(d_b);
@@ -341,7 +365,32 @@ int build_device(pk_ctx *c, const double *dP, double *dJ, double *dK, double *dG
         }
         const int before = nl;
         const ClassFns &f = kClasses[w.cls];
-        (use_os(c, f) ? f.jk_os : f.jk_rys)(a, (w.nwaves + 3) / 4, s, nl);
+        if (use_os(c, f)) {
+            f.jk_os(a, (w.nwaves + 3) / 4, s, nl);
+        } else if (f.has_tmpl) {
+            f.jk_rys(a, (w.nwaves + 3) / 4, s, nl);
+        } else {
+            pk::GenArgs g{};
+            g.wave_bra = a.wave_bra;
+            g.wave_start = a.wave_start;
+            g.wave_count = a.wave_count;
+            g.kets = a.kets;
+            g.pairs = a.pairs;
+            g.prims = a.prims;
+            g.P = a.P;
+            g.Q = a.Q;
+            g.A = a.A;
+            g.B = a.B;
+            g.tol = a.tol;
+            g.nb = nb;
+            g.nitems = w.nwaves;
+            g.la = f.la;
+            g.lb = f.lb;
+            g.lc = f.lc;
+            g.ld = f.ld;
+            if ((rc = launch_generic(g, s)) != PK_OK) return rc;
+            ++nl;
+        }
         w.kernels = nl - before;
         if (c->profile) PK_HIP(hipEventRecord(w.e1, s));
     }
@@ -450,7 +499,10 @@ int pk_ctx_create(const pk_basis_desc *d, pk_ctx **out) {
                     pk::PrimDev pp{};
                     pp.zeta = al + be;
                     const double iz = 1.0 / pp.zeta;
-                    pp.kc = std::pow(M_PI * iz, 1.5) * std::exp(-(al * be * iz) * R2) * ca * cb;
+                    // K_ab sqrt(zeta) (4/pi)^(1/4): two records and 1/sqrt(zeta+eta)
+                    // give K_ab K_cd 2 sqrt(rho/pi)
+                    pp.kc = std::pow(M_PI * iz, 1.5) * std::exp(-(al * be * iz) * R2) * ca * cb *
+                            std::sqrt(pp.zeta) * 1.0622519320271969145;
                     for (int q = 0; q < 3; ++q) {
                         pp.p[q] = (al * Ac[q] + be * Bc[q]) * iz;
                         pp.pa[q] = pp.p[q] - Ac[q];

@@ -9,14 +9,8 @@
     void pk_launch_jk_rys_##a##b##c##d(const pk::JKArgs &x, int g, hipStream_t s, int &n) {   \
         pk::launch_jk_rys<a, b, c, d, 0>(x, g, s, n);                                          \
     }                                                                                          \
-    void pk_launch_eri_rys_##a##b##c##d(const pk::EriArgs &x, int g, hipStream_t s) {          \
-        pk::launch_eri_rys<a, b, c, d, 0>(x, g, s);                                            \
-    }                                                                                          \
     void pk_launch_jk_os_##a##b##c##d(const pk::JKArgs &x, int g, hipStream_t s, int &n) {    \
         pk::launch_jk_os<a, b, c, d>(x, g, s, n);                                              \
-    }                                                                                          \
-    void pk_launch_eri_os_##a##b##c##d(const pk::EriArgs &x, int g, hipStream_t s) {           \
-        pk::launch_eri_os<a, b, c, d>(x, g, s);                                                \
     }
 
 #define PK_SKIP(a, b, c, d)
@@ -60,6 +54,46 @@
 #define PK_DEFINE_IF_7 PK_DEFINE_NOW
 #else
 #define PK_DEFINE_IF_7 PK_SKIP
+#endif
+#if PK_SHARD == 8
+#define PK_DEFINE_IF_8 PK_DEFINE_NOW
+#else
+#define PK_DEFINE_IF_8 PK_SKIP
+#endif
+#if PK_SHARD == 9
+#define PK_DEFINE_IF_9 PK_DEFINE_NOW
+#else
+#define PK_DEFINE_IF_9 PK_SKIP
+#endif
+#if PK_SHARD == 10
+#define PK_DEFINE_IF_10 PK_DEFINE_NOW
+#else
+#define PK_DEFINE_IF_10 PK_SKIP
+#endif
+#if PK_SHARD == 11
+#define PK_DEFINE_IF_11 PK_DEFINE_NOW
+#else
+#define PK_DEFINE_IF_11 PK_SKIP
+#endif
+#if PK_SHARD == 12
+#define PK_DEFINE_IF_12 PK_DEFINE_NOW
+#else
+#define PK_DEFINE_IF_12 PK_SKIP
+#endif
+#if PK_SHARD == 13
+#define PK_DEFINE_IF_13 PK_DEFINE_NOW
+#else
+#define PK_DEFINE_IF_13 PK_SKIP
+#endif
+#if PK_SHARD == 14
+#define PK_DEFINE_IF_14 PK_DEFINE_NOW
+#else
+#define PK_DEFINE_IF_14 PK_SKIP
+#endif
+#if PK_SHARD == 15
+#define PK_DEFINE_IF_15 PK_DEFINE_NOW
+#else
+#define PK_DEFINE_IF_15 PK_SKIP
 #endif
 
 PK_FOR_EACH_CLASS(PK_DEFINE)

@@ -19,6 +19,18 @@ ETHYLENE = ([6, 6, 1, 1, 1, 1],
             np.array([[-3.438338, 0.906062, 0.0], [-3.438338, 2.230667, -0.000001],
                       [-2.836048, 0.343683, -0.709301], [-4.040627, 0.343684, 0.709301],
                       [-2.836049, 2.793045, -0.709302], [-4.040627, 2.793046, 0.709300]]))
+# carbon monoxide, 1.128 A: f shells on two centres (every f class occurs)
+CO = ([6, 8], np.array([[0.0, 0.0, 0.0], [0.0, 0.0, 1.128]]))
+
+
+def _molecule(mol):
+    if mol == "water":
+        return water()
+    if mol == "ethylene":
+        return ETHYLENE
+    if mol == "co":
+        return CO
+    return alkane(3)
 
 
 def _pair(basis, Z, X, engine="os", tol=1e-10):
@@ -35,9 +47,10 @@ def _density(n, seed):
 
 
 @pytest.mark.parametrize("engine", ["os", "rys"])
-@pytest.mark.parametrize("basis,mol", [("sto-3g", "water"), ("6-31g*", "water"), ("cc-pvdz", "ethylene")])
+@pytest.mark.parametrize("basis,mol", [("sto-3g", "water"), ("6-31g*", "water"), ("cc-pvdz", "ethylene"),
+                                      ("cc-pvtz", "water")])
 def test_schwarz_and_eri_tensor(basis, mol, engine):
-    Z, X = water() if mol == "water" else ETHYLENE
+    Z, X = _molecule(mol)
     ref, eng = _pair(basis, Z, X, tol=1e-12)
     eng.set(engine, 1e-12)
     dq = np.abs(eng.schwarz() - ref.matrix("schwarz")).max()
@@ -50,14 +63,10 @@ def test_schwarz_and_eri_tensor(basis, mol, engine):
 
 @pytest.mark.parametrize("engine", ["os", "rys", "auto"])
 @pytest.mark.parametrize("basis,mol,tol", [("6-31g*", "water", 1e-12), ("cc-pvdz", "ethylene", 1e-10),
-                                           ("cc-pvdz", "propane", 1e-12)])
+                                           ("cc-pvdz", "propane", 1e-12), ("cc-pvtz", "water", 1e-12),
+                                           ("cc-pvtz", "co", 1e-12)])
 def test_jk_matches_oracle(basis, mol, tol, engine):
-    if mol == "water":
-        Z, X = water()
-    elif mol == "ethylene":
-        Z, X = ETHYLENE
-    else:
-        Z, X = alkane(3)
+    Z, X = _molecule(mol)
     ref, eng = _pair(basis, Z, X, tol=tol)
     eng.set(engine, tol)
     P = _density(ref.nbasis, 7)

@@ -77,6 +77,7 @@ def test_boys_device_scheme():
     src = open(os.path.join(ENG, "boys_table.inc")).read()
     cols = int(re.search(r"#define PK_BOYS_COLS (\d+)", src).group(1))
     tab = np.array(_arr(src, "pk_boys_table[")).reshape(-1, cols)
+    etab = np.array(_arr(src, "pk_boys_exp["))
 
     def boys(M, T):
         eT = math.exp(-T)
@@ -87,6 +88,10 @@ def test_boys_device_scheme():
             s = row[7]
             for k in range(7, 0, -1):
                 s = s * (-d / k) + row[k - 1]
+            e = 0.0
+            for k in range(8, -1, -1):
+                e = e * (-d) + 1.0 / math.factorial(k)
+            eT = etab[i] * e
             out = [0.0] * (M + 1)
             out[M] = s
             for m in range(M, 0, -1):
