@@ -26,16 +26,26 @@ sys.path.insert(0, os.path.join(ROOT, "planck-refactored_amd"))
 
 METRIC = "J/K Fock builds/s and SCF wall time at |ΔE|≤1e-6 Eh vs plain FP64 GPU"
 PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector / matrix peak
+PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector peak
 
 
-def cpu_baseline(Z, X, seconds, threads):
+WORKLOADS = {
+    # name: (carbons, basis, description)
+    "c20h42-ccpvdz": (20, "cc-pvdz", "RHF J/K build, all-trans n-C20H42 (C-C 1.54 A, C-H 1.09 A, tetrahedral), "
+                                     "cc-pVDZ Cartesian (510 functions), Schwarz 1e-12"),
+    "c10h22-ccpvtz": (10, "cc-pvtz", "RHF J/K build, all-trans n-C10H22 (C-C 1.54 A, C-H 1.09 A, tetrahedral), "
+                                     "cc-pVTZ Cartesian (680 functions, f shells), Schwarz 1e-12"),
+}
+
+
+def cpu_baseline(Z, X, basis, seconds, threads):
     """Oracle Fock build (per-AO-function Obara-Saika, the reference's
     algorithm) on every stride-th bra AO pair, scaled by the stride."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import numpy as np
     from refcpu import Ref
     from pkfock.molecules import ANGSTROM_TO_BOHR
-    ref = Ref("cc-pvdz", Z, X * ANGSTROM_TO_BOHR, engine="os", tol_eri=1e-12)
+    ref = Ref(basis, Z, X * ANGSTROM_TO_BOHR, engine="os", tol_eri=1e-12)
     n = ref.nbasis
     npairs = n * (n + 1) // 2
     rng = np.random.default_rng(0)
@@ -64,6 +74,9 @@ def main():
     ap.add_argument("--no-scf", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="c20h42-ccpvdz")
+    ap.add_argument("--fp32-cutoff", type=float, default=None,
+                    help="mixed precision: quartets with Schwarz bound below this run in FP32")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this engine measures one GPU; multi-GPU runs are not part of this build "
@@ -75,10 +88,15 @@ def main():
     from pkfock.molecules import alkane
 
     dev = torch.device("cuda:0")
-    Z, X = alkane(20)
-    mol = Molecule(Z, X, "cc-pvdz")
+    ncarb, basis, desc = WORKLOADS[args.workload]
+    Z, X = alkane(ncarb)
+    mol = Molecule(Z, X, basis)
     t = time.perf_counter()
     eng = mol.engine("os", tol_eri=1e-12)
+    mixed = args.fp32_cutoff is not None
+    if mixed:
+        from pkfock.engine import PREC_MIXED
+        eng.set("os", 1e-12, precision=PREC_MIXED, fp32_cutoff=args.fp32_cutoff)
     setup_s = time.perf_counter() - t
     n = eng.nbasis
     g = torch.Generator().manual_seed(0)
@@ -93,7 +111,8 @@ def main():
         eng.jk_device(P.data_ptr(), J.data_ptr(), K.data_ptr())
     torch.cuda.synchronize()
     warm_s = time.perf_counter() - t
-    print("bench: C20H42/cc-pVDZ nb=%d setup %.2fs warmup %.2fs" % (n, setup_s, warm_s), file=sys.stderr, flush=True)
+    print("bench: %s nb=%d setup %.2fs warmup %.2fs" % (args.workload, n, setup_s, warm_s), file=sys.stderr,
+          flush=True)
 
     eng.profile(True)
     torch.cuda.synchronize()
@@ -114,12 +133,13 @@ def main():
     print("bench: %d steps %.2f ms/step; model %.3e flop/build = %.2f TFLOP/s whole build" % (
         args.steps, ms, st["model_flops"], st["model_flops"] / ms * 1e-9), file=sys.stderr, flush=True)
     for c in sorted(cls, key=lambda c: -c["ms_avg"]):
-        print("  class (%d%d|%d%d) x%d kernels: %8.3f ms  %.3e quartets  %.2f TFLOP/s" % (
-            c["la"], c["lb"], c["lc"], c["ld"], c["kernels"], c["ms_avg"], c["quartets"],
+        print("  class (%d%d|%d%d)%s x%d kernels: %8.3f ms  %.3e quartets  %.2f TFLOP/s" % (
+            c["la"], c["lb"], c["lc"], c["ld"], " fp32" if c["precision"] else "", c["kernels"], c["ms_avg"],
+            c["quartets"],
             c["model_flops"] / (c["ms_avg"] * 1e-3) * 1e-12), file=sys.stderr, flush=True)
 
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_c20h42_ccpvdz.json")
+    tpath = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.workload.replace("-", "_"))
     if os.path.exists(tpath):
         with open(tpath) as f:
             tr = json.load(f)
@@ -127,7 +147,7 @@ def main():
 
     scf = None
     if not args.no_scf:
-        r = mol.rhf(engine="os", tol_eri=1e-12)
+        r = mol.rhf(engine="os", tol_eri=1e-12, fp32_cutoff=args.fp32_cutoff)
         scf = {"wall_s": round(r["wall_s"], 3), "iterations": r["iterations"], "converged": bool(r["converged"]),
                "energy": r["energy"], "fock_ms_total": round(r["fock_ms"], 1)}
         print("bench: SCF %s" % scf, file=sys.stderr, flush=True)
@@ -136,21 +156,21 @@ def main():
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
         try:
-            cpu = cpu_baseline(Z, X, args.cpu_seconds, threads)
+            cpu = cpu_baseline(Z, X, basis, args.cpu_seconds, threads)
         except Exception as e:  # the baseline is reported, never required
             print("bench: cpu baseline unavailable: %s" % e, file=sys.stderr)
 
+    peak = PEAK_FP32_TFLOPS if dom["precision"] else PEAK_FP64_TFLOPS
     out = {
         "metric": METRIC, "value": args.steps / elapsed, "unit": "builds/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "strong",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "RHF J/K build, all-trans n-C20H42 (C-C 1.54 A, C-H 1.09 A, tetrahedral), "
-                               "cc-pVDZ Cartesian (510 functions), Schwarz 1e-12, FP64",
+        "vs_baseline": None, "dtype": "f64+f32" if mixed else "f64", "data": "synthetic",
+        "config": {"workload": desc + (", mixed FP32 below Schwarz %g" % args.fp32_cutoff if mixed else ", FP64"),
                    "nbasis": n, "shell_quartets": st["quartets"], "primitive_quartets": st["primitive_quartets"],
                    "model_flops_per_build": st["model_flops"],
                    "whole_build_tflops": st["model_flops"] / ms * 1e-9},
-        "roofline": {"bound": "valu", "kernel": dom_name(dom), "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
+        "roofline": {"bound": "valu", "kernel": dom_name(dom), "achieved": achieved, "peak": peak,
+                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": traffic,
                      "kernel_ms": dom["ms_avg"], "kernel_model_flops": dom["model_flops"]},
         "cpu_baseline": cpu,
         "scf": scf,
@@ -159,7 +179,7 @@ def main():
 
 
 def dom_name(c):
-    return "k_jk<%d,%d,%d,%d>" % (c["la"], c["lb"], c["lc"], c["ld"])
+    return "k_jk<%s,%d,%d,%d,%d>" % ("float" if c["precision"] else "double", c["la"], c["lb"], c["lc"], c["ld"])
 
 
 if __name__ == "__main__":

@@ -123,6 +123,7 @@ typedef struct pk_class_stats {
     double model_flops;           /* flop model v1 of one build of this class */
     double ms_total;              /* summed device time over profiled builds */
     int64_t builds;               /* profiled builds */
+    int32_t precision;            /* PK_PREC_FP64, or PK_PREC_MIXED for the FP32 list */
 } pk_class_stats;
 int pk_ctx_profile(pk_ctx *ctx, int enable);
 int pk_ctx_class_count(const pk_ctx *ctx);

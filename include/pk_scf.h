@@ -50,6 +50,8 @@ typedef struct pk_scf_options {
     int32_t charge;
     int32_t use_guess;   /* 1: start from guess_density instead of the core guess */
     const double *guess_density;
+    int32_t precision;   /* PK_PREC_FP64 (default) or PK_PREC_MIXED */
+    double fp32_cutoff;  /* mixed: quartets with Schwarz bound below it run in FP32 */
 } pk_scf_options;
 
 typedef struct pk_scf_result {

@@ -46,52 +46,53 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // Contracted Rys intermediates [e0|cd) of the chunk [C0, C1): acc[NE][NCH].
-template <int LA, int LB, int LC, int LD, int C0, int C1>
+template <typename R, int LA, int LB, int LC, int LD, int C0, int C1>
 __device__ __forceinline__ void rys_contract(const PairDev &pb, const PairDev &pk, int nq, int np,
                                              const PrimDev *__restrict__ prims,
-                                             double (&acc)[Cls<LA, LB, LC, LD>::NE][C1 - C0]) {
+                                             R (&acc)[Cls<LA, LB, LC, LD>::NE][C1 - C0]) {
     using K = Cls<LA, LB, LC, LD>;
-    sfor<0, K::NE>([&](auto i_) { sfor<0, C1 - C0>([&](auto j_) { acc[PK_CI(i_)][PK_CI(j_)] = 0.0; }); });
-    const double cdv[3] = {pk.ab[0], pk.ab[1], pk.ab[2]};
+    sfor<0, K::NE>([&](auto i_) { sfor<0, C1 - C0>([&](auto j_) { acc[PK_CI(i_)][PK_CI(j_)] = R(0); }); });
+    const R cdv[3] = {(R)pk.ab[0], (R)pk.ab[1], (R)pk.ab[2]};
     for (int kq = 0; kq < nq; ++kq) {
         const PrimDev q = prims[pk.prim_off + kq];
         for (int kp = 0; kp < np; ++kp) {
             const PrimDev p = prims[pb.prim_off + kp];
-            rys_prim<LA, LB, LC, LD, C0, C1>(p, q, cdv, acc);
+            rys_prim<R, LA, LB, LC, LD, C0, C1>(p, q, cdv, acc);
         }
     }
 }
 
 // Contracted (ab|cd) of all ket components by OS/HGP: V[NAB][NCD].
-template <int LA, int LB, int LC, int LD>
+template <typename R, int LA, int LB, int LC, int LD>
 __device__ __forceinline__ void os_values(const PairDev &pb, const PairDev &pk, int nq, int np,
                                           const PrimDev *__restrict__ prims,
-                                          double (&V)[Cls<LA, LB, LC, LD>::NAB][Cls<LA, LB, LC, LD>::NCD]) {
+                                          R (&V)[Cls<LA, LB, LC, LD>::NAB][Cls<LA, LB, LC, LD>::NCD]) {
     using K = Cls<LA, LB, LC, LD>;
     constexpr int NE = K::NE, NF = K::NF, NAB = K::NAB;
-    double acc[NE][NF];
-    sfor<0, NE>([&](auto i_) { sfor<0, NF>([&](auto j_) { acc[PK_CI(i_)][PK_CI(j_)] = 0.0; }); });
+    R acc[NE][NF];
+    sfor<0, NE>([&](auto i_) { sfor<0, NF>([&](auto j_) { acc[PK_CI(i_)][PK_CI(j_)] = R(0); }); });
     for (int kq = 0; kq < nq; ++kq) {
         const PrimDev q = prims[pk.prim_off + kq];
         for (int kp = 0; kp < np; ++kp) {
             const PrimDev p = prims[pb.prim_off + kp];
-            os_prim<LA, LB, LC, LD>(p, q, acc);
+            os_prim<R, LA, LB, LC, LD>(p, q, acc);
         }
     }
     // A->B per ket column f, then C->D per bra component ab
-    double Tb[NAB][NF];
+    R Tb[NAB][NF];
     sfor<0, NF>([&](auto jf_) {
         constexpr int jf = PK_CI(jf_);
-        double col[NE], out[NAB];
+        R col[NE], out[NAB];
         sfor<0, NE>([&](auto i_) { col[PK_CI(i_)] = acc[PK_CI(i_)][jf]; });
-        hrr_step<LA, LB, 0>(col, pb.ab, out);
+        hrr_step<R, LA, LB, 0>(col, pb.ab, out);
         sfor<0, NAB>([&](auto i_) { Tb[PK_CI(i_)][jf] = out[PK_CI(i_)]; });
     });
-    sfor<0, NAB>([&](auto i_) { hrr_step<LC, LD, 0>(Tb[PK_CI(i_)], pk.ab, V[PK_CI(i_)]); });
+    sfor<0, NAB>([&](auto i_) { hrr_step<R, LC, LD, 0>(Tb[PK_CI(i_)], pk.ab, V[PK_CI(i_)]); });
 }
 
-// J/K kernel: one wave = one bra pair and up to 64 ket pairs.
-template <int LA, int LB, int LC, int LD, int ALG, int C0, int C1>
+// J/K kernel: one wave = one bra pair and up to 64 ket pairs.  The integrals
+// are computed in R; digestion into J and K is FP64.
+template <typename R, int LA, int LB, int LC, int LD, int ALG, int C0, int C1>
 __global__ __launch_bounds__(256) void k_jk(JKArgs a) {
     using K = Cls<LA, LB, LC, LD>;
     constexpr int NA = K::NA, NB = K::NB, ND = K::ND, NAB = K::NAB, NE = K::NE, NCH = C1 - C0;
@@ -123,7 +124,7 @@ __global__ __launch_bounds__(256) void k_jk(JKArgs a) {
     sfor<0, NB>([&](auto i_) { kbc[PK_CI(i_)] = 0.0; sfor<0, ND>([&](auto d_) { kbd[PK_CI(i_)][PK_CI(d_)] = 0.0; }); });
 
     // digestion of one ket component (ic, id) with its bra block v[NAB]
-    auto digest = [&](auto jj_, const double (&v)[NAB]) {
+    auto digest = [&](auto jj_, const R (&v)[NAB]) {
         constexpr int jj = PK_CI(jj_);
         constexpr int ic = (C0 + jj) / ND, id = (C0 + jj) % ND;
         const int cf = pk.f0 + ic, df = pk.f1 + id;
@@ -139,7 +140,7 @@ __global__ __launch_bounds__(256) void k_jk(JKArgs a) {
                 const int bf = pb.f1 + ib;
                 const size_t abi = (size_t)af * nb + bf;
                 const double m = (all_pass || a.Q[abi] * Qcd >= a.tol) ? deg : 0.0;
-                const double dv = m * v[ia * NB + ib];
+                const double dv = m * (double)v[ia * NB + ib];
                 jcd = fma(dv, P[abi], jcd);
                 jab[ia * NB + ib] = fma(dv, Pcd, jab[ia * NB + ib]);
                 kac[ia] = fma(dv, P[(size_t)bf * nb + df], kac[ia]);
@@ -166,20 +167,20 @@ __global__ __launch_bounds__(256) void k_jk(JKArgs a) {
 
     if constexpr (ALG == ALG_OS) {
         static_assert(C0 == 0 && C1 == K::NCD, "the OS kernels take all ket components");
-        double V[NAB][K::NCD];
-        os_values<LA, LB, LC, LD>(pb, pk, nq, pb.nprim, a.prims, V);
+        R V[NAB][K::NCD];
+        os_values<R, LA, LB, LC, LD>(pb, pk, nq, pb.nprim, a.prims, V);
         sfor<0, NCH>([&](auto jj_) {
-            double v[NAB];
+            R v[NAB];
             sfor<0, NAB>([&](auto i_) { v[PK_CI(i_)] = V[PK_CI(i_)][PK_CI(jj_)]; });
             digest(jj_, v);
         });
     } else {
-        double acc[NE][NCH];
-        rys_contract<LA, LB, LC, LD, C0, C1>(pb, pk, nq, pb.nprim, a.prims, acc);
+        R acc[NE][NCH];
+        rys_contract<R, LA, LB, LC, LD, C0, C1>(pb, pk, nq, pb.nprim, a.prims, acc);
         sfor<0, NCH>([&](auto jj_) {
-            double col[NE], v[NAB];
+            R col[NE], v[NAB];
             sfor<0, NE>([&](auto i_) { col[PK_CI(i_)] = acc[PK_CI(i_)][PK_CI(jj_)]; });
-            hrr_step<LA, LB, 0>(col, pb.ab, v);
+            hrr_step<R, LA, LB, 0>(col, pb.ab, v);
             digest(jj_, v);
         });
     }

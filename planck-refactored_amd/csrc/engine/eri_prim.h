@@ -1,7 +1,10 @@
-// eri_prim.h — per-primitive-quartet building blocks of the ERI kernels (FP64):
-// the Boys function, the Rys rule, the Obara-Saika / Head-Gordon-Pople vertical
+// eri_prim.h — per-primitive-quartet building blocks of the ERI kernels: the
+// Boys function, the Rys rule, the Obara-Saika / Head-Gordon-Pople vertical
 // recurrence and the Rys 2-D integrals, unrolled per angular class at compile
-// time (sfor), so every intermediate lives in registers.
+// time (sfor), so every intermediate lives in registers.  Each block is a
+// template over the arithmetic type R: double for the FP64 path, float for the
+// quartets of the mixed-precision path whose Schwarz bound lies below the FP32
+// cutoff (the Boys large-T branch of those stays FP64).
 //
 // Reference semantics restated (the contracted Cartesian ERIs are the same
 // numbers, computed in another order):
@@ -33,6 +36,30 @@ __device__ __forceinline__ double rsqrt_nr(double x) {
     y = fma(y, fma(-h * y, y, 0.5), y);
     y = fma(y, fma(-h * y, y, 0.5), y);
     return y;
+}
+__device__ __forceinline__ double rsq(double x) { return rsqrt_nr(x); }
+__device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+
+// scalars of one primitive quartet (p on the bra, q on the ket)
+template <typename R>
+struct PrimQ {
+    R zeta, eta, rs, inv_de, rho, PQ[3], T, pref;
+};
+
+template <typename R>
+__device__ __forceinline__ PrimQ<R> prim_quartet(const PrimDev &p, const PrimDev &q) {
+    PrimQ<R> s;
+    s.zeta = (R)p.zeta;
+    s.eta = (R)q.zeta;
+    s.rs = rsq(s.zeta + s.eta);
+    s.inv_de = s.rs * s.rs;
+    s.rho = s.zeta * s.eta * s.inv_de;
+    s.PQ[0] = (R)(p.p[0] - q.p[0]);
+    s.PQ[1] = (R)(p.p[1] - q.p[1]);
+    s.PQ[2] = (R)(p.p[2] - q.p[2]);
+    s.T = s.rho * fma(s.PQ[0], s.PQ[0], fma(s.PQ[1], s.PQ[1], s.PQ[2] * s.PQ[2]));
+    s.pref = (R)(p.kc * q.kc) * s.rs;
+    return s;
 }
 
 // ---------------------------------------------------------------- Boys
@@ -85,51 +112,98 @@ __device__ __forceinline__ void boys(double T, double (&F)[M + 1]) {
     }
 }
 
+// FP32 path: 5-term series from the float tables below 36 (|d| <= 1/32, next
+// term below 3e-10 relative); the large-T branch in FP64.
+template <int M>
+__device__ __forceinline__ void boys(float T, float (&F)[M + 1]) {
+    static_assert(M + 5 <= PK_BOYS_COLS, "Boys table too narrow for this class");
+    if (T < (float)PK_BOYS_XMAX) {
+        const int i = (int)fmaf(T, (float)PK_BOYS_STEP_INV, 0.5f);
+        const float d = fmaf((float)i, -1.0f / PK_BOYS_STEP_INV, T);
+        const float *row = pk_boys_table_f + i * PK_BOYS_COLS + M;
+        float s = row[4];
+        s = fmaf(s, -d * (1.0f / 4.0f), row[3]);
+        s = fmaf(s, -d * (1.0f / 3.0f), row[2]);
+        s = fmaf(s, -d * (1.0f / 2.0f), row[1]);
+        s = fmaf(s, -d, row[0]);
+        F[M] = s;
+        if constexpr (M > 0) {
+            float e = fmaf(-d, 1.0f / 120.0f, 1.0f / 24.0f);
+            e = fmaf(e, -d, 1.0f / 6.0f);
+            e = fmaf(e, -d, 0.5f);
+            e = fmaf(e, -d, 1.0f);
+            e = fmaf(e, -d, 1.0f);
+            const float eT = pk_boys_exp_f[i] * e;
+            const float t2 = 2.0f * T;
+            sfor_down<M, 0>([&](auto m_) {
+                constexpr int m = PK_CI(m_);
+                F[m - 1] = fmaf(t2, F[m], eT) * (1.0f / (2 * m - 1));
+            });
+        }
+    } else {
+        const double Td = (double)T;
+        const double r = rsqrt_nr(Td);
+        double f = 0.88622692545275801365 * r;
+        F[0] = (float)f;
+        if constexpr (M > 0) {
+            const double eT = (double)__expf(-T), it2 = 0.5 * r * r;
+            sfor<1, M + 1>([&](auto m_) {
+                constexpr int m = PK_CI(m_);
+                f = ((2 * m - 1) * f - eT) * it2;
+                F[m] = (float)f;
+            });
+        }
+    }
+}
+
 // ---------------------------------------------------------------- Rys rule
 // Root r of the n-point rule of e^{-T u}/(2 sqrt u) du on [0,1]: Chebyshev fits
 // below TMAX(n), the Gauss-Hermite limit above.
+template <typename R>
 struct RysCtx {
     const double *c;  // coefficients of T's interval (null in the large-T limit)
-    double x, it, sit;
+    R x, it, sit;
 };
 
-template <int N>
-__device__ __forceinline__ RysCtx rys_ctx(double T) {
-    RysCtx k;
-    if (T >= pk_rys_tmax[N]) {
+template <typename R, int N>
+__device__ __forceinline__ RysCtx<R> rys_ctx(R T) {
+    RysCtx<R> k;
+    if (T >= (R)pk_rys_tmax[N]) {
         k.c = nullptr;
-        k.sit = rsqrt_nr(T);
+        k.sit = rsq(T);
         k.it = k.sit * k.sit;
-        k.x = 0.0;
+        k.x = R(0);
     } else {
         const int i = (int)T;
         k.c = pk_rys_coef + pk_rys_base[N] + i * (2 * N * (PK_RYS_DEG + 1));
-        k.x = 2.0 * (T - (double)i) - 1.0;
-        k.it = k.sit = 0.0;
+        k.x = R(2) * (T - (R)i) - R(1);
+        k.it = k.sit = R(0);
     }
     return k;
 }
 
-__device__ __forceinline__ double cheb(const double *cf, double x) {
-    const double x2 = 2.0 * x;
-    double b1 = 0.0, b2 = 0.0;
+// Clenshaw sum of a Chebyshev series (coefficients FP64, arithmetic in R)
+template <typename R>
+__device__ __forceinline__ R cheb(const double *cf, R x) {
+    const R x2 = R(2) * x;
+    R b1 = R(0), b2 = R(0);
 #pragma unroll
     for (int j = PK_RYS_DEG; j >= 1; --j) {
-        const double t = fma(x2, b1, cf[j] - b2);
+        const R t = fma(x2, b1, (R)cf[j] - b2);
         b2 = b1;
         b1 = t;
     }
-    return fma(x, b1, cf[0] - b2);
+    return fma(x, b1, (R)cf[0] - b2);
 }
 
-template <int N>
-__device__ __forceinline__ void rys_root(const RysCtx &k, int r, double &u, double &w) {
+template <typename R, int N>
+__device__ __forceinline__ void rys_root(const RysCtx<R> &k, int r, R &u, R &w) {
     if (k.c == nullptr) {
-        u = pk_rys_ax2[pk_rys_abase[N] + r] * k.it;
-        w = pk_rys_aw[pk_rys_abase[N] + r] * k.sit;
+        u = (R)pk_rys_ax2[pk_rys_abase[N] + r] * k.it;
+        w = (R)pk_rys_aw[pk_rys_abase[N] + r] * k.sit;
     } else {
-        u = cheb(k.c + r * (PK_RYS_DEG + 1), k.x);
-        w = cheb(k.c + (N + r) * (PK_RYS_DEG + 1), k.x);
+        u = cheb<R>(k.c + r * (PK_RYS_DEG + 1), k.x);
+        w = cheb<R>(k.c + (N + r) * (PK_RYS_DEG + 1), k.x);
     }
 }
 
@@ -159,25 +233,21 @@ __host__ __device__ constexpr int flat0_shift(int l, int k, int q, int s) {
 // quartet into acc[NE][NF].  V[e][f][m] holds [e0|f0]^(m) for all |e| <= E,
 // |f| <= F (flat level-0 indices): bra side first, then the ket side with the
 // bra-ket coupling term (Obara & Saika 1986, Head-Gordon & Pople 1988).
-template <int LA, int LB, int LC, int LD>
+template <typename R, int LA, int LB, int LC, int LD>
 __device__ __forceinline__ void os_prim(const PrimDev &p, const PrimDev &q,
-                                        double (&acc)[Cls<LA, LB, LC, LD>::NE][Cls<LA, LB, LC, LD>::NF]) {
+                                        R (&acc)[Cls<LA, LB, LC, LD>::NE][Cls<LA, LB, LC, LD>::NF]) {
     using K = Cls<LA, LB, LC, LD>;
     constexpr int E = K::E, F = K::F, L = K::L;
     constexpr int NLE = nlev(0, E), NLF = nlev(0, F);
-    const double zeta = p.zeta, eta = q.zeta;
-    const double rs = rsqrt_nr(zeta + eta);
-    const double inv_de = rs * rs;
-    const double rho = zeta * eta * inv_de;
-    const double PQ[3] = {p.p[0] - q.p[0], p.p[1] - q.p[1], p.p[2] - q.p[2]};
-    const double T = rho * fma(PQ[0], PQ[0], fma(PQ[1], PQ[1], PQ[2] * PQ[2]));
-    const double pref = p.kc * q.kc * rs;
-    double Fm[L + 1];
-    boys<L>(T, Fm);
-    double V[NLE][NLF][L + 1];
-    sfor<0, L + 1>([&](auto m_) { V[0][0][PK_CI(m_)] = pref * Fm[PK_CI(m_)]; });
-    const double fp = -eta * inv_de, fq = zeta * inv_de;  // W - P = fp PQ, W - Q = fq PQ
-    const double i2z = 0.5 / zeta, roz = rho / zeta, i2e = 0.5 / eta, roe = rho / eta, i2d = 0.5 * inv_de;
+    const PrimQ<R> s = prim_quartet<R>(p, q);
+    const R zeta = s.zeta, eta = s.eta, inv_de = s.inv_de, rho = s.rho;
+    const R PQ[3] = {s.PQ[0], s.PQ[1], s.PQ[2]};
+    R Fm[L + 1];
+    boys<L>(s.T, Fm);
+    R V[NLE][NLF][L + 1];
+    sfor<0, L + 1>([&](auto m_) { V[0][0][PK_CI(m_)] = s.pref * Fm[PK_CI(m_)]; });
+    const R fp = -eta * inv_de, fq = zeta * inv_de;  // W - P = fp PQ, W - Q = fq PQ
+    const R i2z = R(0.5) / zeta, roz = rho / zeta, i2e = R(0.5) / eta, roe = rho / eta, i2d = R(0.5) * inv_de;
     sfor<1, E + 1>([&](auto le_) {
         constexpr int le = PK_CI(le_);
         sfor<0, ncart(le)>([&](auto k_) {
@@ -186,13 +256,13 @@ __device__ __forceinline__ void os_prim(const PrimDev &p, const PrimDev &q,
             constexpr int ei = comp_q(le, k, ax);
             constexpr int e = flat(0, cart_x(le, k), cart_y(le, k), cart_z(le, k));
             constexpr int e1 = flat0_shift(le, k, ax, -1);
-            const double pa = p.pa[ax], wp = fp * PQ[ax];
+            const R pa = (R)p.pa[ax], wp = fp * PQ[ax];
             sfor<0, L - le + 1>([&](auto m_) {
                 constexpr int m = PK_CI(m_);
-                double t = fma(pa, V[e1][0][m], wp * V[e1][0][m + 1]);
+                R t = fma(pa, V[e1][0][m], wp * V[e1][0][m + 1]);
                 if constexpr (ei >= 2) {
                     constexpr int e2 = flat0_shift(le, k, ax, -2);
-                    t = fma((ei - 1) * i2z, fma(-roz, V[e2][0][m + 1], V[e2][0][m]), t);
+                    t = fma(R(ei - 1) * i2z, fma(-roz, V[e2][0][m + 1], V[e2][0][m]), t);
                 }
                 V[e][0][m] = t;
             });
@@ -206,7 +276,7 @@ __device__ __forceinline__ void os_prim(const PrimDev &p, const PrimDev &q,
             constexpr int fi = comp_q(lf, kf, ax);
             constexpr int f = flat(0, cart_x(lf, kf), cart_y(lf, kf), cart_z(lf, kf));
             constexpr int f1 = flat0_shift(lf, kf, ax, -1);
-            const double qc = q.pa[ax], wq = fq * PQ[ax];
+            const R qc = (R)q.pa[ax], wq = fq * PQ[ax];
             sfor<0, E + 1>([&](auto le_) {
                 constexpr int le = PK_CI(le_);
                 sfor<0, ncart(le)>([&](auto ke_) {
@@ -215,14 +285,14 @@ __device__ __forceinline__ void os_prim(const PrimDev &p, const PrimDev &q,
                     constexpr int ei = comp_q(le, ke, ax);
                     sfor<0, L - le - lf + 1>([&](auto m_) {
                         constexpr int m = PK_CI(m_);
-                        double t = fma(qc, V[e][f1][m], wq * V[e][f1][m + 1]);
+                        R t = fma(qc, V[e][f1][m], wq * V[e][f1][m + 1]);
                         if constexpr (fi >= 2) {
                             constexpr int f2 = flat0_shift(lf, kf, ax, -2);
-                            t = fma((fi - 1) * i2e, fma(-roe, V[e][f2][m + 1], V[e][f2][m]), t);
+                            t = fma(R(fi - 1) * i2e, fma(-roe, V[e][f2][m + 1], V[e][f2][m]), t);
                         }
                         if constexpr (ei >= 1) {
                             constexpr int em = flat0_shift(le, ke, ax, -1);
-                            t = fma(ei * i2d, V[em][f1][m + 1], t);
+                            t = fma(R(ei) * i2d, V[em][f1][m + 1], t);
                         }
                         V[e][f][m] = t;
                     });
@@ -241,29 +311,29 @@ __device__ __forceinline__ void os_prim(const PrimDev &p, const PrimDev &q,
 // ---------------------------------------------------------------- Rys 2-D integrals
 // G[e][c][d] of one axis: 2-D recurrence (e <= E, f <= LC+LD) then the C->D
 // transfer on the ket index.
-template <int E, int LC, int LD>
-__device__ __forceinline__ void axis_2d(double seed, double C00, double D00, double B00, double B10, double B01,
-                                        double CDq, double (&G)[E + 1][LC + 1][LD + 1]) {
+template <typename R, int E, int LC, int LD>
+__device__ __forceinline__ void axis_2d(R seed, R C00, R D00, R B00, R B10, R B01, R CDq,
+                                        R (&G)[E + 1][LC + 1][LD + 1]) {
     constexpr int F = LC + LD;
-    double I[E + 1][F + 1];
+    R I[E + 1][F + 1];
     I[0][0] = seed;
     sfor<1, E + 1>([&](auto e_) {
         constexpr int e = PK_CI(e_);
         I[e][0] = C00 * I[e - 1][0];
-        if constexpr (e >= 2) I[e][0] = fma((double)(e - 1) * B10, I[e - 2][0], I[e][0]);
+        if constexpr (e >= 2) I[e][0] = fma(R(e - 1) * B10, I[e - 2][0], I[e][0]);
     });
     sfor<1, F + 1>([&](auto f_) {
         constexpr int f = PK_CI(f_);
         I[0][f] = D00 * I[0][f - 1];
-        if constexpr (f >= 2) I[0][f] = fma((double)(f - 1) * B01, I[0][f - 2], I[0][f]);
+        if constexpr (f >= 2) I[0][f] = fma(R(f - 1) * B01, I[0][f - 2], I[0][f]);
         sfor<1, E + 1>([&](auto e_) {
             constexpr int e = PK_CI(e_);
-            double t = fma(D00, I[e][f - 1], (double)e * B00 * I[e - 1][f - 1]);
-            if constexpr (f >= 2) t = fma((double)(f - 1) * B01, I[e][f - 2], t);
+            R t = fma(D00, I[e][f - 1], R(e) * B00 * I[e - 1][f - 1]);
+            if constexpr (f >= 2) t = fma(R(f - 1) * B01, I[e][f - 2], t);
             I[e][f] = t;
         });
     });
-    double H[E + 1][F + 1][LD + 1];
+    R H[E + 1][F + 1][LD + 1];
     sfor<0, E + 1>([&](auto e_) { sfor<0, F + 1>([&](auto c_) { H[PK_CI(e_)][PK_CI(c_)][0] = I[PK_CI(e_)][PK_CI(c_)]; }); });
     sfor<1, LD + 1>([&](auto d_) {
         constexpr int d = PK_CI(d_);
@@ -286,33 +356,30 @@ __device__ __forceinline__ void axis_2d(double seed, double C00, double D00, dou
 
 // One primitive quartet by Rys quadrature: accumulate [e0|cd) of the ket
 // components [C0, C1) into acc[NE][C1 - C0]; one root per loop trip.
-template <int LA, int LB, int LC, int LD, int C0, int C1>
-__device__ __forceinline__ void rys_prim(const PrimDev &p, const PrimDev &q, const double cd[3],
-                                         double (&acc)[Cls<LA, LB, LC, LD>::NE][C1 - C0]) {
+template <typename R, int LA, int LB, int LC, int LD, int C0, int C1>
+__device__ __forceinline__ void rys_prim(const PrimDev &p, const PrimDev &q, const R cd[3],
+                                         R (&acc)[Cls<LA, LB, LC, LD>::NE][C1 - C0]) {
     using K = Cls<LA, LB, LC, LD>;
     constexpr int NR = K::NR, E = K::E;
-    const double zeta = p.zeta, eta = q.zeta;
-    const double rs = rsqrt_nr(zeta + eta);
-    const double inv_de = rs * rs;
-    const double rho = zeta * eta * inv_de;
-    const double PQx = p.p[0] - q.p[0], PQy = p.p[1] - q.p[1], PQz = p.p[2] - q.p[2];
-    const double T = rho * fma(PQx, PQx, fma(PQy, PQy, PQz * PQz));
-    const double fp = -eta * inv_de, fq = zeta * inv_de;
-    const double pref = p.kc * q.kc * rs;
-    const double i2z = 0.5 / zeta, i2e = 0.5 / eta, roz = rho / zeta, roe = rho / eta;
-    const RysCtx rk = rys_ctx<NR>(T);
+    const PrimQ<R> s = prim_quartet<R>(p, q);
+    const R zeta = s.zeta, eta = s.eta, inv_de = s.inv_de, rho = s.rho;
+    const R PQx = s.PQ[0], PQy = s.PQ[1], PQz = s.PQ[2];
+    const R fp = -eta * inv_de, fq = zeta * inv_de;
+    const R i2z = R(0.5) / zeta, i2e = R(0.5) / eta, roz = rho / zeta, roe = rho / eta;
+    const RysCtx<R> rk = rys_ctx<R, NR>(s.T);
 #pragma unroll 1
     for (int r = 0; r < NR; ++r) {
-        double ur, wr;
-        rys_root<NR>(rk, r, ur, wr);
-        const double B00 = 0.5 * inv_de * ur;
-        const double B10 = i2z * (1.0 - roz * ur);
-        const double B01 = i2e * (1.0 - roe * ur);
-        const double uwp = ur * fp, uwq = ur * fq;
-        double Gx[E + 1][LC + 1][LD + 1], Gy[E + 1][LC + 1][LD + 1], Gz[E + 1][LC + 1][LD + 1];
-        axis_2d<E, LC, LD>(pref * wr, fma(uwp, PQx, p.pa[0]), fma(uwq, PQx, q.pa[0]), B00, B10, B01, cd[0], Gx);
-        axis_2d<E, LC, LD>(1.0, fma(uwp, PQy, p.pa[1]), fma(uwq, PQy, q.pa[1]), B00, B10, B01, cd[1], Gy);
-        axis_2d<E, LC, LD>(1.0, fma(uwp, PQz, p.pa[2]), fma(uwq, PQz, q.pa[2]), B00, B10, B01, cd[2], Gz);
+        R ur, wr;
+        rys_root<R, NR>(rk, r, ur, wr);
+        const R B00 = R(0.5) * inv_de * ur;
+        const R B10 = i2z * (R(1) - roz * ur);
+        const R B01 = i2e * (R(1) - roe * ur);
+        const R uwp = ur * fp, uwq = ur * fq;
+        R Gx[E + 1][LC + 1][LD + 1], Gy[E + 1][LC + 1][LD + 1], Gz[E + 1][LC + 1][LD + 1];
+        axis_2d<R, E, LC, LD>(s.pref * wr, fma(uwp, PQx, (R)p.pa[0]), fma(uwq, PQx, (R)q.pa[0]), B00, B10, B01, cd[0],
+                              Gx);
+        axis_2d<R, E, LC, LD>(R(1), fma(uwp, PQy, (R)p.pa[1]), fma(uwq, PQy, (R)q.pa[1]), B00, B10, B01, cd[1], Gy);
+        axis_2d<R, E, LC, LD>(R(1), fma(uwp, PQz, (R)p.pa[2]), fma(uwq, PQz, (R)q.pa[2]), B00, B10, B01, cd[2], Gz);
         sfor<0, K::NE>([&](auto ie_) {
             constexpr int ie = PK_CI(ie_);
             constexpr int le = flat_level(LA, ie), ke = flat_k(LA, ie);
@@ -332,21 +399,21 @@ __device__ __forceinline__ void rys_prim(const PrimDev &p, const PrimDev &q, con
 // A -> B transfer: cur over (a levels LA..LA+LB-K) x (b level K) -> out (a, b)
 // of levels (LA, LB).  Used for the bra (A -> B) and, with the ket's angular
 // momenta, for the ket (C -> D).
-template <int LA, int LB, int K>
-__device__ __forceinline__ void hrr_step(const double (&cur)[nlev(LA, LA + LB - K) * ncart(K)], const double ab[3],
-                                         double (&out)[ncart(LA) * ncart(LB)]) {
+template <typename R, int LA, int LB, int K>
+__device__ __forceinline__ void hrr_step(const R (&cur)[nlev(LA, LA + LB - K) * ncart(K)], const double ab[3],
+                                         R (&out)[ncart(LA) * ncart(LB)]) {
     if constexpr (K == LB) {
         sfor<0, ncart(LA) * ncart(LB)>([&](auto i_) { out[PK_CI(i_)] = cur[PK_CI(i_)]; });
     } else {
         constexpr int NK = ncart(K), NK1 = ncart(K + 1);
         constexpr int NAL = nlev(LA, LA + LB - K - 1);
-        double nxt[NAL * NK1];
+        R nxt[NAL * NK1];
         sfor<0, NK1>([&](auto bp_) {
             constexpr int bp = PK_CI(bp_);
             constexpr int bx = cart_x(K + 1, bp), by = cart_y(K + 1, bp), bz = cart_z(K + 1, bp);
             constexpr int ax = build_axis(bx, by, bz);
             constexpr int b = cart_index(bx - (ax == 0), by - (ax == 1), bz - (ax == 2));
-            const double abq = ab[ax];
+            const R abq = (R)ab[ax];
             sfor<0, NAL>([&](auto ia_) {
                 constexpr int ia = PK_CI(ia_);
                 constexpr int la = flat_level(LA, ia), ka = flat_k(LA, ia);
@@ -355,7 +422,7 @@ __device__ __forceinline__ void hrr_step(const double (&cur)[nlev(LA, LA + LB - 
                 nxt[ia * NK1 + bp] = fma(abq, cur[ia * NK + b], cur[iap * NK + b]);
             });
         });
-        hrr_step<LA, LB, K + 1>(nxt, ab, out);
+        hrr_step<R, LA, LB, K + 1>(nxt, ab, out);
     }
 }
 

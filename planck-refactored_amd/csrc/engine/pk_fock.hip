@@ -26,7 +26,9 @@
 // per-class launchers, defined in pk_kernels.hip
 #define PK_DECL(a, b, c, d, shard)                                                             \
     void pk_launch_jk_rys_##a##b##c##d(const pk::JKArgs &, int, hipStream_t, int &);           \
-    void pk_launch_jk_os_##a##b##c##d(const pk::JKArgs &, int, hipStream_t, int &);
+    void pk_launch_jk_os_##a##b##c##d(const pk::JKArgs &, int, hipStream_t, int &);            \
+    void pk_launch_jk_rys32_##a##b##c##d(const pk::JKArgs &, int, hipStream_t, int &);         \
+    void pk_launch_jk_os32_##a##b##c##d(const pk::JKArgs &, int, hipStream_t, int &);
 PK_FOR_EACH_CLASS(PK_DECL)
 #undef PK_DECL
 
@@ -56,11 +58,14 @@ struct ClassFns {
     bool has_tmpl;  // Rys per-lane kernel (else the cooperative kernel)
     void (*jk_os)(const pk::JKArgs &, int, hipStream_t, int &);
     void (*jk_rys)(const pk::JKArgs &, int, hipStream_t, int &);
+    void (*jk_os32)(const pk::JKArgs &, int, hipStream_t, int &);  // FP32 integrals
+    void (*jk_rys32)(const pk::JKArgs &, int, hipStream_t, int &);
 };
 
 #define PK_ENTRY(a, b, c, d, shard)                                                             \
     ClassFns{a, b, c, d, pk::os_available(a, b, c, d), pk::rys_tmpl_available(a, b, c, d),      \
-             &pk_launch_jk_os_##a##b##c##d, &pk_launch_jk_rys_##a##b##c##d},
+             &pk_launch_jk_os_##a##b##c##d, &pk_launch_jk_rys_##a##b##c##d,                     \
+             &pk_launch_jk_os32_##a##b##c##d, &pk_launch_jk_rys32_##a##b##c##d},
 // Quartet classes (bra class >= ket class), indexed cb (cb + 1) / 2 + ck with
 // pair classes (0,0) (1,0) (1,1) (2,0) (2,1) (2,2) (3,0) (3,1) (3,2) (3,3).
 const ClassFns kClasses[kQuartClasses] = {PK_FOR_EACH_CLASS(PK_ENTRY)};
@@ -105,6 +110,7 @@ __global__ void k_finalize(const double *__restrict__ A, const double *__restric
 
 struct ClassWork {
     int cls = 0;
+    int prec = PK_PREC_FP64;  // PK_PREC_MIXED: the FP32 list of the class
     int nwaves = 0;
     int64_t nquart = 0;
     int64_t nprimq = 0;
@@ -255,7 +261,9 @@ int build_schwarz(pk_ctx *c) {
 
 // Screened work lists: for every quartet class (cb >= ck) and every bra pair,
 // the ket pairs with qmax_bra * qmax_ket >= tol (and ket <= bra when cb == ck),
-// sorted by primitive count, cut into waves of at most 64 kets.
+// sorted by primitive count, cut into waves of at most 64 kets.  In mixed
+// precision the kets with qmax_bra * qmax_ket < cutoff form a second, FP32
+// list of the class (classes of the cooperative kernel stay FP64).
 int build_work(pk_ctx *c) {
     free_work(c);
     const int np = (int)c->pairs.size();
@@ -266,61 +274,74 @@ int build_work(pk_ctx *c) {
     c->st.quartets = 0;
     c->st.primitive_quartets = 0;
     c->st.model_flops = 0.0;
+    const bool mixed = c->precision == PK_PREC_MIXED;
     for (int cb = 0; cb < kPairClasses; ++cb)
         for (int ck = 0; ck <= cb; ++ck) {
-            ClassWork w;
-            w.cls = quart_class(cb, ck);
-            const ClassFns &f = kClasses[w.cls];
+            const int cls = quart_class(cb, ck);
+            const ClassFns &f = kClasses[cls];
             const std::vector<int> &B = by_cls[cb], &Kl = by_cls[ck];
             if (B.empty() || Kl.empty()) continue;
-            std::vector<int> wb, ws, wc, kets;
-            std::vector<int> tmp;
-            for (size_t ib = 0; ib < B.size(); ++ib) {
-                const int b = B[ib];
-                const double qb = c->pair_q[b];
-                // number of kets in q-descending order passing the test
-                size_t lo = 0, hi = Kl.size();
-                while (lo < hi) {
-                    const size_t mid = (lo + hi) / 2;
-                    if (qb * c->pair_q[Kl[mid]] >= c->tol)
-                        lo = mid + 1;
-                    else
-                        hi = mid;
+            const bool split = mixed && f.has_tmpl;
+            for (int prec : {PK_PREC_FP64, PK_PREC_MIXED}) {
+                if (prec == PK_PREC_MIXED && !split) continue;
+                ClassWork w;
+                w.cls = cls;
+                w.prec = prec;
+                std::vector<int> wb, ws, wc, kets;
+                std::vector<int> tmp;
+                for (size_t ib = 0; ib < B.size(); ++ib) {
+                    const int b = B[ib];
+                    const double qb = c->pair_q[b];
+                    // number of kets (q descending) with qb * qk >= thr
+                    auto count = [&](double thr) {
+                        size_t lo = 0, hi = Kl.size();
+                        while (lo < hi) {
+                            const size_t mid = (lo + hi) / 2;
+                            if (qb * c->pair_q[Kl[mid]] >= thr)
+                                lo = mid + 1;
+                            else
+                                hi = mid;
+                        }
+                        return cb == ck ? std::min(lo, ib + 1) : lo;
+                    };
+                    const size_t n_tol = count(c->tol);
+                    const size_t n_hi = split ? std::min(count(c->cutoff), n_tol) : n_tol;
+                    const size_t k0 = prec == PK_PREC_FP64 ? 0 : n_hi;
+                    const size_t k1 = prec == PK_PREC_FP64 ? n_hi : n_tol;
+                    const size_t n = k1 - k0;
+                    if (n == 0) continue;
+                    tmp.assign(Kl.begin() + k0, Kl.begin() + k1);
+                    std::stable_sort(tmp.begin(), tmp.end(),
+                                     [&](int x, int y) { return c->pairs[x].nprim > c->pairs[y].nprim; });
+                    const int kb = c->pairs[b].nprim;
+                    for (size_t s = 0; s < n; s += 64) {
+                        wb.push_back(b);
+                        ws.push_back((int)kets.size() + (int)s);
+                        wc.push_back((int)std::min<size_t>(64, n - s));
+                    }
+                    for (size_t s = 0; s < n; ++s) {
+                        const double K = (double)kb * c->pairs[tmp[s]].nprim;
+                        w.nprimq += (int64_t)K;
+                        w.flops += pk::model_flops(f.la, f.lb, f.lc, f.ld, K);
+                    }
+                    kets.insert(kets.end(), tmp.begin(), tmp.end());
+                    w.nquart += (int64_t)n;
                 }
-                size_t n = lo;
-                if (cb == ck) n = std::min(n, ib + 1);
-                if (n == 0) continue;
-                tmp.assign(Kl.begin(), Kl.begin() + n);
-                std::stable_sort(tmp.begin(), tmp.end(),
-                                 [&](int x, int y) { return c->pairs[x].nprim > c->pairs[y].nprim; });
-                const int kb = c->pairs[b].nprim;
-                for (size_t s = 0; s < n; s += 64) {
-                    wb.push_back(b);
-                    ws.push_back((int)kets.size() + (int)s);
-                    wc.push_back((int)std::min<size_t>(64, n - s));
-                }
-                for (size_t s = 0; s < n; ++s) {
-                    const double K = (double)kb * c->pairs[tmp[s]].nprim;
-                    w.nprimq += (int64_t)K;
-                    w.flops += pk::model_flops(f.la, f.lb, f.lc, f.ld, K);
-                }
-                kets.insert(kets.end(), tmp.begin(), tmp.end());
-                w.nquart += (int64_t)n;
+                if (w.nquart == 0) continue;
+                w.nwaves = (int)wb.size();
+                std::vector<int> packed;
+                packed.reserve(3 * wb.size());
+                packed.insert(packed.end(), wb.begin(), wb.end());
+                packed.insert(packed.end(), ws.begin(), ws.end());
+                packed.insert(packed.end(), wc.begin(), wc.end());
+                int rc;
+                if ((rc = dev_upload(&w.d_wave, packed, c->stream)) != PK_OK) return rc;
+                if ((rc = dev_upload(&w.d_kets, kets, c->stream)) != PK_OK) return rc;
+                c->st.quartets += w.nquart;
+                c->st.primitive_quartets += w.nprimq;
+                c->st.model_flops += w.flops;
+                c->work.push_back(w);
             }
-            if (w.nquart == 0) continue;
-            w.nwaves = (int)wb.size();
-            std::vector<int> packed;
-            packed.reserve(3 * wb.size());
-            packed.insert(packed.end(), wb.begin(), wb.end());
-            packed.insert(packed.end(), ws.begin(), ws.end());
-            packed.insert(packed.end(), wc.begin(), wc.end());
-            int rc;
-            if ((rc = dev_upload(&w.d_wave, packed, c->stream)) != PK_OK) return rc;
-            if ((rc = dev_upload(&w.d_kets, kets, c->stream)) != PK_OK) return rc;
-            c->st.quartets += w.nquart;
-            c->st.primitive_quartets += w.nprimq;
-            c->st.model_flops += w.flops;
-            c->work.push_back(w);
         }
     // largest classes first
     std::sort(c->work.begin(), c->work.end(), [](const ClassWork &x, const ClassWork &y) { return x.flops > y.flops; });
@@ -365,10 +386,11 @@ int build_device(pk_ctx *c, const double *dP, double *dJ, double *dK, double *dG
         }
         const int before = nl;
         const ClassFns &f = kClasses[w.cls];
+        const bool fp32 = w.prec == PK_PREC_MIXED;
         if (use_os(c, f)) {
-            f.jk_os(a, (w.nwaves + 3) / 4, s, nl);
+            (fp32 ? f.jk_os32 : f.jk_os)(a, (w.nwaves + 3) / 4, s, nl);
         } else if (f.has_tmpl) {
-            f.jk_rys(a, (w.nwaves + 3) / 4, s, nl);
+            (fp32 ? f.jk_rys32 : f.jk_rys)(a, (w.nwaves + 3) / 4, s, nl);
         } else {
             pk::GenArgs g{};
             g.wave_bra = a.wave_bra;
@@ -560,13 +582,16 @@ int pk_ctx_set(pk_ctx *c, int engine, double tol, int precision, double cutoff) 
     if (!c) return fail(PK_ERR_ARG, "pk_ctx_set: null context");
     if (engine < PK_ENGINE_OS || engine > PK_ENGINE_AUTO) return fail(PK_ERR_ARG, "pk_ctx_set: unknown engine");
     if (!(tol >= 0.0)) return fail(PK_ERR_ARG, "pk_ctx_set: tol_eri must be >= 0");
-    if (precision != PK_PREC_FP64)
-        return fail(PK_ERR_UNSUPPORTED, "pk_ctx_set: only PK_PREC_FP64 is available in this build");
+    if (precision != PK_PREC_FP64 && precision != PK_PREC_MIXED)
+        return fail(PK_ERR_ARG, "pk_ctx_set: unknown precision");
+    if (precision == PK_PREC_MIXED && !(cutoff >= 0.0))
+        return fail(PK_ERR_ARG, "pk_ctx_set: the FP32 cutoff must be >= 0");
+    if (precision == PK_PREC_FP64) cutoff = 0.0;
     c->engine = engine;
-    c->precision = precision;
-    c->cutoff = cutoff;
-    if (tol != c->tol) {
+    if (tol != c->tol || precision != c->precision || cutoff != c->cutoff) {
         c->tol = tol;
+        c->precision = precision;
+        c->cutoff = cutoff;
         c->work_ok = false;
     }
     return PK_OK;
@@ -615,6 +640,7 @@ int pk_ctx_class_stats(const pk_ctx *c, int i, pk_class_stats *o) {
     o->model_flops = w.flops;
     o->ms_total = w.ms_total;
     o->builds = w.builds;
+    o->precision = w.prec;
     return PK_OK;
 }
 

@@ -187,6 +187,8 @@ void pk_scf_default_options(pk_scf_options *o) {
     o->charge = 0;
     o->use_guess = 0;
     o->guess_density = nullptr;
+    o->precision = PK_PREC_FP64;
+    o->fp32_cutoff = 0.0;
 }
 
 int pk_scf_rhf(const pk_basis *bas, const pk_scf_options *opt, pk_scf_result *res, double *P_out, double *hist,
@@ -252,7 +254,7 @@ int pk_scf_rhf(const pk_basis *bas, const pk_scf_options *opt, pk_scf_result *re
     pk_basis_describe(bas, &desc);
     SCF_PK(pk_ctx_create(&desc, &ctx));
     SCF_PK(pk_ctx_set_stream(ctx, st));
-    SCF_PK(pk_ctx_set(ctx, opt->engine, opt->tol_eri, PK_PREC_FP64, 0.0));
+    SCF_PK(pk_ctx_set(ctx, opt->engine, opt->tol_eri, opt->precision, opt->fp32_cutoff));
 
     S = dev.alloc(n2), H = dev.alloc(n2), X = dev.alloc(n2), P = dev.alloc(n2), Pn = dev.alloc(n2);
     F = dev.alloc(n2), G = dev.alloc(n2), M = dev.alloc(n2), D = dev.alloc(n2), W1 = dev.alloc(n2), C = dev.alloc(n2);

@@ -22,7 +22,8 @@ class ScfOptions(ctypes.Structure):
     _fields_ = [("tol_energy", ctypes.c_double), ("tol_density", ctypes.c_double), ("tol_eri", ctypes.c_double),
                 ("diis_dim", ctypes.c_int32), ("max_cycles", ctypes.c_int32), ("engine", ctypes.c_int32),
                 ("charge", ctypes.c_int32), ("use_guess", ctypes.c_int32),
-                ("guess_density", ctypes.POINTER(ctypes.c_double))]
+                ("guess_density", ctypes.POINTER(ctypes.c_double)), ("precision", ctypes.c_int32),
+                ("fp32_cutoff", ctypes.c_double)]
 
 
 class ScfResult(ctypes.Structure):
@@ -110,11 +111,15 @@ class Molecule:
         return _eng.FockEngine(self.shells(), engine=engine, tol_eri=tol_eri)
 
     def rhf(self, engine="os", tol_eri=1e-10, tol_energy=1e-10, tol_density=1e-10, diis_dim=8, max_cycles=0,
-            guess=None):
+            guess=None, fp32_cutoff=None):
+        """Device RHF (pk_scf_rhf).  fp32_cutoff: mixed precision, quartets
+        with Schwarz bound below it in FP32 (None: FP64 throughout)."""
         lib = load()
         o = ScfOptions()
         lib.pk_scf_default_options(ctypes.byref(o))
         o.tol_energy, o.tol_density, o.tol_eri = tol_energy, tol_density, tol_eri
+        if fp32_cutoff is not None:
+            o.precision, o.fp32_cutoff = _eng.PREC_MIXED, float(fp32_cutoff)
         o.diis_dim, o.max_cycles = diis_dim, max_cycles
         o.engine = _eng.ENGINE[engine] if isinstance(engine, str) else int(engine)
         o.charge = self.charge

@@ -78,3 +78,30 @@ def test_jk_matches_oracle(basis, mol, tol, engine):
     assert dj < FOCK_TOL and dk < FOCK_TOL
     G = eng.g(P)
     assert np.abs(G - (Jr - 0.5 * Kr)).max() < FOCK_TOL
+
+
+@pytest.mark.parametrize("cutoff", [1e-4, 1e-6])
+@pytest.mark.parametrize("basis,mol", [("cc-pvdz", "propane"), ("cc-pvtz", "water")])
+def test_mixed_precision_jk(basis, mol, cutoff):
+    """Mixed precision: quartets with Schwarz bound below the cutoff in FP32.
+    Each FP32 integral carries a relative error of a few 1e-7 on a value below
+    the cutoff, so J and K stay within 1e-6 * cutoff * (number of terms) of
+    FP64; the bar here is 1e-5 * cutoff * nb^2 / 100, and FP64 quartets only
+    (cutoff 0) must still meet the FP64 bar."""
+    from pkfock.engine import PREC_MIXED
+    Z, X = _molecule(mol)
+    ref, eng = _pair(basis, Z, X, tol=1e-12)
+    P = _density(ref.nbasis, 11)
+    Jr, Kr = ref.jk(P)
+    eng.set("os", 1e-12, precision=PREC_MIXED, fp32_cutoff=0.0)
+    J, K = eng.jk(P)
+    assert max(np.abs(J - Jr).max(), np.abs(K - Kr).max()) < FOCK_TOL
+    eng.set("os", 1e-12, precision=PREC_MIXED, fp32_cutoff=cutoff)
+    J, K = eng.jk(P)
+    dj, dk = np.abs(J - Jr).max(), np.abs(K - Kr).max()
+    n32 = sum(c["quartets"] for c in eng.class_stats() if c["precision"])
+    print("%s/%s cutoff %g: %d FP32 quartets of %d  max|dJ| = %.3e  max|dK| = %.3e" % (
+        mol, basis, cutoff, n32, eng.stats()["quartets"], dj, dk))
+    assert n32 > 0
+    bar = 1e-5 * cutoff * ref.nbasis ** 2 / 100
+    assert dj < bar and dk < bar
