@@ -88,6 +88,29 @@ typedef struct pk_stats {
 int pk_ctx_create(const pk_basis_desc *basis, pk_ctx **out);
 int pk_ctx_destroy(pk_ctx *ctx);
 
+/* Batch of nmol independent molecules in one context: the batched form of
+ * the reference's per-molecule _compute_2e_fock (base.h:57-73), one call
+ * building every molecule's J and K.  Matrices are packed: molecule m's
+ * nb_m x nb_m block starts at element offset[m] (pk_ctx_mol_layout); the
+ * quartet lists of all molecules are merged per angular class, so one build
+ * is one kernel per class for the whole batch.  pk_ctx_create is the batch of
+ * one.  pk_fock_jk / pk_fock_2e / pk_fock_jk_uhf / pk_schwarz / pk_ctx_set
+ * work on batches unchanged (packed arrays); pk_eri_tensor takes one molecule. */
+int pk_batch_create(const pk_basis_desc *basis, int nmol, pk_ctx **out);
+int pk_ctx_nmol(const pk_ctx *ctx);
+int pk_ctx_mol_layout(const pk_ctx *ctx, int32_t *nb, int64_t *offset);
+
+/* hipGraph mode: the kernels of a build are captured once and replayed on
+ * every later build (inputs and outputs copied through the context's own
+ * buffers).  Off by default; recaptured after any change of the work lists. */
+int pk_ctx_set_graph(pk_ctx *ctx, int enable);
+
+/* Launch-shape autotuner: times every class list under each candidate shape
+ * (1, 2 or 4 waves per workgroup; with or without XCD-contiguous wave ranges)
+ * over `reps` builds and keeps the fastest per class for this context.  The
+ * density buffer is overwritten. */
+int pk_ctx_autotune(pk_ctx *ctx, int reps);
+
 /* Engine and threshold (types.h:407-411).  precision PK_PREC_MIXED computes
  * quartets whose Schwarz bound is below fp32_cutoff in FP32 (accumulation in
  * FP64); PK_PREC_FP64 ignores fp32_cutoff. */
@@ -124,6 +147,8 @@ typedef struct pk_class_stats {
     double ms_total;              /* summed device time over profiled builds */
     int64_t builds;               /* profiled builds */
     int32_t precision;            /* PK_PREC_FP64, or PK_PREC_MIXED for the FP32 list */
+    int32_t block;                /* lanes per workgroup of the class kernels */
+    int32_t remap;                /* 1: XCD-contiguous wave ranges */
 } pk_class_stats;
 int pk_ctx_profile(pk_ctx *ctx, int enable);
 int pk_ctx_class_count(const pk_ctx *ctx);

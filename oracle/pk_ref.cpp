@@ -1224,16 +1224,20 @@ int pkref_fock_jk_sample(pkref_ctx *c, const double *P, double *J, double *K, in
     const int nb = c->nb;
     const long np = (long)c->pairs.size();
     const int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
-    std::vector<std::vector<double>> Aacc(nt), Bacc(nt);
+    // The sampled bra pairs are dealt round-robin to a fixed number of slots,
+    // each summed serially into its own accumulators and the slots added in
+    // order: the result is the same for every thread count and every run.
+    const long nbra = np > offset ? (np - offset + stride - 1) / stride : 0;
+    const int nslot = (int)std::min<long>(64, std::max<long>(1, nbra));
+    std::vector<std::vector<double>> Aacc(nslot), Bacc(nslot);
     double count = 0.0;
-#pragma omp parallel num_threads(nt) reduction(+ : count)
-    {
-        const int t = omp_get_thread_num();
-        std::vector<double> &A = Aacc[t], &B = Bacc[t];
+#pragma omp parallel for num_threads(nt) schedule(dynamic) reduction(+ : count)
+    for (int sl = 0; sl < nslot; ++sl) {
+        std::vector<double> &A = Aacc[sl], &B = Bacc[sl];
         A.assign((size_t)nb * nb, 0.0);
         B.assign((size_t)nb * nb, 0.0);
-#pragma omp for schedule(dynamic)
-        for (long p = offset; p < np; p += stride) {
+        for (long ib = sl; ib < nbra; ib += nslot) {
+            const long p = offset + ib * stride;
             const Pair &ab = c->pairs[p];
             const size_t i = ab.i, j = ab.j;
             const double qij = c->Q[i * nb + j];
@@ -1257,9 +1261,13 @@ int pkref_fock_jk_sample(pkref_ctx *c, const double *P, double *J, double *K, in
             }
         }
     }
+    if (nbra == 0) {
+        Aacc[0].assign((size_t)nb * nb, 0.0);
+        Bacc[0].assign((size_t)nb * nb, 0.0);
+    }
     for (size_t x = 0; x < (size_t)nb * nb; ++x) {
         double a = 0, b = 0;
-        for (int t = 0; t < nt; ++t) {
+        for (int t = 0; t < nslot; ++t) {
             a += Aacc[t][x];
             b += Bacc[t][x];
         }

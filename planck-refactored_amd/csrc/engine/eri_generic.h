@@ -30,7 +30,10 @@ struct GenArgs {
     const int *__restrict__ wave_bra;
     const int *__restrict__ wave_start;
     const int *__restrict__ wave_count;
+    const int *__restrict__ wave_mol;
     const int *__restrict__ kets;
+    const long long *__restrict__ mol_off;  // JK mode: packed matrices per molecule
+    const int *__restrict__ mol_nb;
     const int *__restrict__ qbra;  // ERI mode
     const int *__restrict__ qket;
     const PairDev *__restrict__ pairs;
@@ -41,7 +44,6 @@ struct GenArgs {
     double *B;
     double *out;  // ERI mode: [nitems][NA NB NC ND]
     double tol;
-    int nb;
     int nitems;  // waves (JK) or quartets (ERI)
     int la, lb, lc, ld;
     int eri_mode;
@@ -91,11 +93,12 @@ __global__ __launch_bounds__(64) void k_generic(GenArgs a) {
     const int NC = (lc + 1) * (lc + 2) / 2, ND = (ld + 1) * (ld + 2) / 2;
     const int NCD = NC * ND, NABCD = NA * NB * NCD;
     const int E = la + lb, F = lc + ld, n = (E + F) / 2 + 1;
-    const int nb = a.nb;
     double *G = smem;               // [n][3][GEN_G]
     double *acc = G + n * 3 * GEN_G;  // [NABCD]
 
-    int bra, kbeg, kcnt;
+    int bra, kbeg, kcnt, nb = 0;
+    const double *P = nullptr, *Qm = nullptr;
+    double *Am = nullptr, *Bm = nullptr;
     if (a.eri_mode) {
         bra = a.qbra[item];
         kbeg = item;
@@ -104,6 +107,13 @@ __global__ __launch_bounds__(64) void k_generic(GenArgs a) {
         bra = a.wave_bra[item];
         kbeg = a.wave_start[item];
         kcnt = a.wave_count[item];
+        const int mol = a.wave_mol[item];
+        const long long moff = a.mol_off[mol];
+        nb = a.mol_nb[mol];
+        P = a.P + moff;
+        Qm = a.Q + moff;
+        Am = a.A + moff;
+        Bm = a.B + moff;
     }
     const PairDev pb = a.pairs[bra];
 
@@ -231,23 +241,23 @@ __global__ __launch_bounds__(64) void k_generic(GenArgs a) {
             if (!all_pass) {
                 const int cd = i % NCD, ab = i / NCD;
                 const int af = pb.f0 + ab / NB, bf = pb.f1 + ab % NB, cf = pk.f0 + cd / ND, df = pk.f1 + cd % ND;
-                if (a.Q[(size_t)af * nb + bf] * a.Q[(size_t)cf * nb + df] < a.tol) m = 0.0;
+                if (Qm[(size_t)af * nb + bf] * Qm[(size_t)cf * nb + df] < a.tol) m = 0.0;
             }
             acc[i] *= m;
         }
         __syncthreads();
-        auto Pv = [&](int r, int c) { return a.P[(size_t)r * nb + c]; };
+        auto Pv = [&](int r, int c) { return P[(size_t)r * nb + c]; };
         // A_ab += 2 sum_cd v P_cd
         for (int ab = lane; ab < NA * NB; ab += 64) {
             double s = 0.0;
             for (int cd = 0; cd < NCD; ++cd) s = fma(acc[ab * NCD + cd], Pv(pk.f0 + cd / ND, pk.f1 + cd % ND), s);
-            unsafeAtomicAdd(&a.A[(size_t)(pb.f0 + ab / NB) * nb + pb.f1 + ab % NB], 2.0 * s);
+            unsafeAtomicAdd(&Am[(size_t)(pb.f0 + ab / NB) * nb + pb.f1 + ab % NB], 2.0 * s);
         }
         // A_cd += 2 sum_ab v P_ab
         for (int cd = lane; cd < NCD; cd += 64) {
             double s = 0.0;
             for (int ab = 0; ab < NA * NB; ++ab) s = fma(acc[ab * NCD + cd], Pv(pb.f0 + ab / NB, pb.f1 + ab % NB), s);
-            unsafeAtomicAdd(&a.A[(size_t)(pk.f0 + cd / ND) * nb + pk.f1 + cd % ND], 2.0 * s);
+            unsafeAtomicAdd(&Am[(size_t)(pk.f0 + cd / ND) * nb + pk.f1 + cd % ND], 2.0 * s);
         }
         // B_ac += sum_bd v P_bd ; B_bc += sum_ad v P_ad ; B_ad += sum_bc v P_bc ; B_bd += sum_ac v P_ac
         for (int t = lane; t < NA * NC; t += 64) {
@@ -256,7 +266,7 @@ __global__ __launch_bounds__(64) void k_generic(GenArgs a) {
             for (int ib = 0; ib < NB; ++ib)
                 for (int id = 0; id < ND; ++id)
                     s = fma(acc[(ia * NB + ib) * NCD + ic * ND + id], Pv(pb.f1 + ib, pk.f1 + id), s);
-            unsafeAtomicAdd(&a.B[(size_t)(pb.f0 + ia) * nb + pk.f0 + ic], s);
+            unsafeAtomicAdd(&Bm[(size_t)(pb.f0 + ia) * nb + pk.f0 + ic], s);
         }
         for (int t = lane; t < NB * NC; t += 64) {
             const int ib = t / NC, ic = t % NC;
@@ -264,7 +274,7 @@ __global__ __launch_bounds__(64) void k_generic(GenArgs a) {
             for (int ia = 0; ia < NA; ++ia)
                 for (int id = 0; id < ND; ++id)
                     s = fma(acc[(ia * NB + ib) * NCD + ic * ND + id], Pv(pb.f0 + ia, pk.f1 + id), s);
-            unsafeAtomicAdd(&a.B[(size_t)(pb.f1 + ib) * nb + pk.f0 + ic], s);
+            unsafeAtomicAdd(&Bm[(size_t)(pb.f1 + ib) * nb + pk.f0 + ic], s);
         }
         for (int t = lane; t < NA * ND; t += 64) {
             const int ia = t / ND, id = t % ND;
@@ -272,7 +282,7 @@ __global__ __launch_bounds__(64) void k_generic(GenArgs a) {
             for (int ib = 0; ib < NB; ++ib)
                 for (int ic = 0; ic < NC; ++ic)
                     s = fma(acc[(ia * NB + ib) * NCD + ic * ND + id], Pv(pb.f1 + ib, pk.f0 + ic), s);
-            unsafeAtomicAdd(&a.B[(size_t)(pb.f0 + ia) * nb + pk.f1 + id], s);
+            unsafeAtomicAdd(&Bm[(size_t)(pb.f0 + ia) * nb + pk.f1 + id], s);
         }
         for (int t = lane; t < NB * ND; t += 64) {
             const int ib = t / ND, id = t % ND;
@@ -280,7 +290,7 @@ __global__ __launch_bounds__(64) void k_generic(GenArgs a) {
             for (int ia = 0; ia < NA; ++ia)
                 for (int ic = 0; ic < NC; ++ic)
                     s = fma(acc[(ia * NB + ib) * NCD + ic * ND + id], Pv(pb.f0 + ia, pk.f0 + ic), s);
-            unsafeAtomicAdd(&a.B[(size_t)(pb.f1 + ib) * nb + pk.f1 + id], s);
+            unsafeAtomicAdd(&Bm[(size_t)(pb.f1 + ib) * nb + pk.f1 + id], s);
         }
         __syncthreads();
     }

@@ -23,21 +23,40 @@ namespace pk {
 
 enum { ALG_OS = 0, ALG_RYS = 1 };
 
+// Work of one class: waves (bra pair, ket range, molecule).  The matrices of
+// molecule m start at mol_off[m] of the packed arrays, leading dimension
+// mol_nb[m]; a single molecule is the batch of one.
 struct JKArgs {
     const int *__restrict__ wave_bra;
     const int *__restrict__ wave_start;
     const int *__restrict__ wave_count;
+    const int *__restrict__ wave_mol;
     const int *__restrict__ kets;
     const PairDev *__restrict__ pairs;
     const PrimDev *__restrict__ prims;
+    const long long *__restrict__ mol_off;
+    const int *__restrict__ mol_nb;
     const double *__restrict__ P;  // density scaled by component norms
     const double *__restrict__ Q;  // Schwarz per function pair (normalised functions)
     double *A;                     // J accumulator, one orientation
     double *B;                     // K accumulator, one orientation
     double tol;
-    int nb;
     int nwaves;
+    int remap;  // 1: each XCD takes a contiguous range of waves (grid a multiple of 8)
 };
+
+// Wave handled by this wave slot.  With remap, workgroup b (dispatched to XCD
+// b % 8) takes logical block (b % 8) * (grid / 8) + b / 8, a bijection on the
+// grid when the grid is a multiple of 8, so consecutive waves (same bra pair,
+// neighbouring kets) share one XCD's L2.
+__device__ __forceinline__ int wave_index(int remap) {
+    int blk = blockIdx.x;
+    if (remap) {
+        const int per = gridDim.x >> 3;
+        blk = (blk & 7) * per + (blk >> 3);
+    }
+    return __builtin_amdgcn_readfirstlane(blk * (int)(blockDim.x >> 6) + (int)(threadIdx.x >> 6));
+}
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -96,8 +115,15 @@ template <typename R, int LA, int LB, int LC, int LD, int ALG, int C0, int C1>
 __global__ __launch_bounds__(256) void k_jk(JKArgs a) {
     using K = Cls<LA, LB, LC, LD>;
     constexpr int NA = K::NA, NB = K::NB, ND = K::ND, NAB = K::NAB, NE = K::NE, NCH = C1 - C0;
-    const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+    const int wave = wave_index(a.remap);
     if (wave >= a.nwaves) return;
+    const int mol = a.wave_mol[wave];
+    const long long moff = a.mol_off[mol];
+    const int nb = a.mol_nb[mol];
+    const double *__restrict__ P = a.P + moff;
+    const double *__restrict__ Qm = a.Q + moff;
+    double *Am = a.A + moff;
+    double *Bm = a.B + moff;
     const int lane = threadIdx.x & 63;
     const int bra = a.wave_bra[wave];
     const int cnt = a.wave_count[wave];
@@ -115,8 +141,6 @@ __global__ __launch_bounds__(256) void k_jk(JKArgs a) {
     // every function quartet passes the Schwarz test when the smallest
     // function-pair bounds do; otherwise each one is tested
     const bool all_pass = pb.qmin * pk.qmin >= a.tol;
-    const int nb = a.nb;
-    const double *__restrict__ P = a.P;
 
     double jab[NAB], kac[NA], kbc[NB], kad[NA][ND], kbd[NB][ND];
     sfor<0, NAB>([&](auto i_) { jab[PK_CI(i_)] = 0.0; });
@@ -129,7 +153,7 @@ __global__ __launch_bounds__(256) void k_jk(JKArgs a) {
         constexpr int ic = (C0 + jj) / ND, id = (C0 + jj) % ND;
         const int cf = pk.f0 + ic, df = pk.f1 + id;
         const double Pcd = P[(size_t)cf * nb + df];
-        const double Qcd = a.Q[(size_t)cf * nb + df];
+        const double Qcd = Qm[(size_t)cf * nb + df];
         double jcd = 0.0;
         sfor<0, NA>([&](auto ia_) {
             constexpr int ia = PK_CI(ia_);
@@ -139,7 +163,7 @@ __global__ __launch_bounds__(256) void k_jk(JKArgs a) {
                 constexpr int ib = PK_CI(ib_);
                 const int bf = pb.f1 + ib;
                 const size_t abi = (size_t)af * nb + bf;
-                const double m = (all_pass || a.Q[abi] * Qcd >= a.tol) ? deg : 0.0;
+                const double m = (all_pass || Qm[abi] * Qcd >= a.tol) ? deg : 0.0;
                 const double dv = m * (double)v[ia * NB + ib];
                 jcd = fma(dv, P[abi], jcd);
                 jab[ia * NB + ib] = fma(dv, Pcd, jab[ia * NB + ib]);
@@ -149,15 +173,15 @@ __global__ __launch_bounds__(256) void k_jk(JKArgs a) {
                 kbd[ib][id] = fma(dv, Pac, kbd[ib][id]);
             });
         });
-        if (active) unsafeAtomicAdd(&a.A[(size_t)cf * nb + df], 2.0 * jcd);
+        if (active) unsafeAtomicAdd(&Am[(size_t)cf * nb + df], 2.0 * jcd);
         // last ket component of this c in the chunk: flush the (a|c), (b|c) sums
         if constexpr (id == ND - 1 || jj == NCH - 1) {
             if (active) {
                 sfor<0, NA>([&](auto ia_) {
-                    unsafeAtomicAdd(&a.B[(size_t)(pb.f0 + PK_CI(ia_)) * nb + cf], kac[PK_CI(ia_)]);
+                    unsafeAtomicAdd(&Bm[(size_t)(pb.f0 + PK_CI(ia_)) * nb + cf], kac[PK_CI(ia_)]);
                 });
                 sfor<0, NB>([&](auto ib_) {
-                    unsafeAtomicAdd(&a.B[(size_t)(pb.f1 + PK_CI(ib_)) * nb + cf], kbc[PK_CI(ib_)]);
+                    unsafeAtomicAdd(&Bm[(size_t)(pb.f1 + PK_CI(ib_)) * nb + cf], kbc[PK_CI(ib_)]);
                 });
             }
             sfor<0, NA>([&](auto i_) { kac[PK_CI(i_)] = 0.0; });
@@ -195,10 +219,10 @@ __global__ __launch_bounds__(256) void k_jk(JKArgs a) {
             if constexpr (present) {
                 const int df = pk.f1 + id;
                 sfor<0, NA>([&](auto ia_) {
-                    unsafeAtomicAdd(&a.B[(size_t)(pb.f0 + PK_CI(ia_)) * nb + df], kad[PK_CI(ia_)][id]);
+                    unsafeAtomicAdd(&Bm[(size_t)(pb.f0 + PK_CI(ia_)) * nb + df], kad[PK_CI(ia_)][id]);
                 });
                 sfor<0, NB>([&](auto ib_) {
-                    unsafeAtomicAdd(&a.B[(size_t)(pb.f1 + PK_CI(ib_)) * nb + df], kbd[PK_CI(ib_)][id]);
+                    unsafeAtomicAdd(&Bm[(size_t)(pb.f1 + PK_CI(ib_)) * nb + df], kbd[PK_CI(ib_)][id]);
                 });
             }
         });
@@ -206,7 +230,7 @@ __global__ __launch_bounds__(256) void k_jk(JKArgs a) {
     sfor<0, NAB>([&](auto i_) {
         constexpr int i = PK_CI(i_);
         const double s = wave_sum(jab[i]);
-        if (lane == 0) unsafeAtomicAdd(&a.A[(size_t)(pb.f0 + i / NB) * nb + pb.f1 + i % NB], 2.0 * s);
+        if (lane == 0) unsafeAtomicAdd(&Am[(size_t)(pb.f0 + i / NB) * nb + pb.f1 + i % NB], 2.0 * s);
     });
 }
 

@@ -25,10 +25,10 @@
 
 // per-class launchers, defined in pk_kernels.hip
 #define PK_DECL(a, b, c, d, shard)                                                             \
-    void pk_launch_jk_rys_##a##b##c##d(const pk::JKArgs &, int, hipStream_t, int &);           \
-    void pk_launch_jk_os_##a##b##c##d(const pk::JKArgs &, int, hipStream_t, int &);            \
-    void pk_launch_jk_rys32_##a##b##c##d(const pk::JKArgs &, int, hipStream_t, int &);         \
-    void pk_launch_jk_os32_##a##b##c##d(const pk::JKArgs &, int, hipStream_t, int &);
+    void pk_launch_jk_rys_##a##b##c##d(const pk::JKArgs &, int, int, hipStream_t, int &);           \
+    void pk_launch_jk_os_##a##b##c##d(const pk::JKArgs &, int, int, hipStream_t, int &);            \
+    void pk_launch_jk_rys32_##a##b##c##d(const pk::JKArgs &, int, int, hipStream_t, int &);         \
+    void pk_launch_jk_os32_##a##b##c##d(const pk::JKArgs &, int, int, hipStream_t, int &);
 PK_FOR_EACH_CLASS(PK_DECL)
 #undef PK_DECL
 
@@ -56,10 +56,10 @@ struct ClassFns {
     int la, lb, lc, ld;
     bool has_os;    // Obara-Saika per-lane kernel
     bool has_tmpl;  // Rys per-lane kernel (else the cooperative kernel)
-    void (*jk_os)(const pk::JKArgs &, int, hipStream_t, int &);
-    void (*jk_rys)(const pk::JKArgs &, int, hipStream_t, int &);
-    void (*jk_os32)(const pk::JKArgs &, int, hipStream_t, int &);  // FP32 integrals
-    void (*jk_rys32)(const pk::JKArgs &, int, hipStream_t, int &);
+    void (*jk_os)(const pk::JKArgs &, int, int, hipStream_t, int &);
+    void (*jk_rys)(const pk::JKArgs &, int, int, hipStream_t, int &);
+    void (*jk_os32)(const pk::JKArgs &, int, int, hipStream_t, int &);  // FP32 integrals
+    void (*jk_rys32)(const pk::JKArgs &, int, int, hipStream_t, int &);
 };
 
 #define PK_ENTRY(a, b, c, d, shard)                                                             \
@@ -80,12 +80,23 @@ int class_la(int c) {
 int class_lb(int c) { return c - class_la(c) * (class_la(c) + 1) / 2; }
 
 // ---------------------------------------------------------------- small kernels
-__global__ void k_scale_density(const double *__restrict__ P, const double *__restrict__ n, double *__restrict__ Pn,
-                                int nb) {
+// Packed matrices: molecule m (blockIdx.y) has nb = mol_nb[m] functions, its
+// nb x nb block at mol_off[m], its component norms at nrm[mol_foff[m]].
+struct MolTab {
+    const long long *off;
+    const int *nb;
+    const int *foff;
+};
+
+__global__ void k_scale_density(const double *__restrict__ P, const double *__restrict__ nrm,
+                                double *__restrict__ Pn, MolTab t) {
+    const int m = blockIdx.y, nb = t.nb[m];
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (size_t)nb * nb) return;
     const int r = (int)(i / nb), c = (int)(i % nb);
-    Pn[i] = n[r] * n[c] * P[i];
+    const double *n = nrm + t.foff[m];
+    const size_t o = (size_t)t.off[m] + i;
+    Pn[o] = n[r] * n[c] * P[o];
 }
 
 __global__ void k_fill(double *__restrict__ x, size_t n, double v) {
@@ -94,28 +105,36 @@ __global__ void k_fill(double *__restrict__ x, size_t n, double v) {
 }
 
 // J = n n^T o (A + A^T), K = n n^T o (B + B^T); optional G = J - K/2
-__global__ void k_finalize(const double *__restrict__ A, const double *__restrict__ B, const double *__restrict__ n,
-                           double *J, double *K, double *G, int nb) {
+__global__ void k_finalize(const double *__restrict__ A, const double *__restrict__ B,
+                           const double *__restrict__ nrm, double *J, double *K, double *G, MolTab t) {
+    const int m = blockIdx.y, nb = t.nb[m];
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (size_t)nb * nb) return;
     const int r = (int)(i / nb), c = (int)(i % nb);
-    const size_t t = (size_t)c * nb + r;
+    const double *n = nrm + t.foff[m];
+    const size_t o = (size_t)t.off[m];
     const double s = n[r] * n[c];
-    const double j = s * (A[i] + A[t]);
-    const double k = s * (B[i] + B[t]);
-    if (J) J[i] = j;
-    if (K) K[i] = k;
-    if (G) G[i] = j - 0.5 * k;
+    const double j = s * (A[o + i] + A[o + (size_t)c * nb + r]);
+    const double k = s * (B[o + i] + B[o + (size_t)c * nb + r]);
+    if (J) J[o + i] = j;
+    if (K) K[o + i] = k;
+    if (G) G[o + i] = j - 0.5 * k;
 }
+
+struct TuneEntry {
+    int cls, prec, block, remap;
+};
 
 struct ClassWork {
     int cls = 0;
     int prec = PK_PREC_FP64;  // PK_PREC_MIXED: the FP32 list of the class
+    int block = 256;          // lanes per workgroup (autotuned)
+    int remap = 0;            // XCD-contiguous wave ranges (autotuned)
     int nwaves = 0;
     int64_t nquart = 0;
     int64_t nprimq = 0;
     double flops = 0.0;
-    int *d_wave = nullptr;  // [3][nwaves]: bra, start, count
+    int *d_wave = nullptr;  // [4][nwaves]: bra, start, count, molecule
     int *d_kets = nullptr;
     int kernels = 0;
     double ms_total = 0.0;
@@ -135,7 +154,20 @@ int dev_upload(T **dst, const std::vector<T> &src, hipStream_t s) {
 }  // namespace
 
 struct pk_ctx {
-    int nb = 0, ns = 0;
+    int nb = 0, ns = 0;  // functions and shells over all molecules
+    int nmol = 1;
+    std::vector<int> mol_nb, mol_foff;  // functions per molecule, first function
+    std::vector<long long> mol_off;     // offset of the molecule's nb x nb block
+    size_t n2 = 0;                      // packed matrix length
+    int nbmax = 0;
+    std::vector<int> pair_mol;
+    long long *d_mol_off = nullptr;
+    int *d_mol_nb = nullptr, *d_mol_foff = nullptr;
+    bool graph_on = false, graph_ok = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    double *d_G = nullptr;
+    std::vector<TuneEntry> tuned;  // autotuned launch shape per (class, precision)
     std::vector<int> sh_l, sh_np, sh_poff, sh_f0;
     std::vector<double> sh_c, ex, co, pn;
     std::vector<double> fnorm;
@@ -143,7 +175,7 @@ struct pk_ctx {
     std::vector<int> pair_cls;
     std::vector<double> pair_q;
     std::vector<pk::PrimDev> prims;
-    std::vector<double> Q;  // nb*nb, normalised functions
+    std::vector<double> Q;  // packed nb x nb blocks, normalised functions
     pk::PairDev *d_pairs = nullptr;
     pk::PrimDev *d_prims = nullptr;
     double *d_Q = nullptr, *d_fnorm = nullptr, *d_P = nullptr, *d_Pn = nullptr, *d_A = nullptr, *d_B = nullptr;
@@ -166,6 +198,14 @@ namespace {
 // OS kernels where they exist unless the Rys engine is forced
 bool use_os(const pk_ctx *c, const ClassFns &f) { return f.has_os && c->engine != PK_ENGINE_RYS; }
 
+void drop_graph(pk_ctx *c) {
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->gexec = nullptr;
+    c->graph = nullptr;
+    c->graph_ok = false;
+}
+
 void free_work(pk_ctx *c) {
     for (ClassWork &w : c->work) {
         if (w.d_wave) (void)hipFree(w.d_wave);
@@ -175,6 +215,7 @@ void free_work(pk_ctx *c) {
     }
     c->work.clear();
     c->work_ok = false;
+    drop_graph(c);
 }
 
 // Raw contracted ERIs (no component norms) of the quartets (bra[i] | ket[i]),
@@ -228,8 +269,7 @@ int run_eri_class(pk_ctx *c, int cb, int ck, const std::vector<int> &bra, const 
 }
 
 int build_schwarz(pk_ctx *c) {
-    const int nb = c->nb;
-    c->Q.assign((size_t)nb * nb, 0.0);
+    c->Q.assign(c->n2, 0.0);
     for (int cls = 0; cls < kPairClasses; ++cls) {
         std::vector<int> idx;
         for (int p = 0; p < (int)c->pairs.size(); ++p)
@@ -242,13 +282,16 @@ int build_schwarz(pk_ctx *c) {
         const int na = pk::ncart(la), nbb = pk::ncart(lb), nab = na * nbb;
         for (size_t q = 0; q < idx.size(); ++q) {
             const pk::PairDev &p = c->pairs[idx[q]];
+            const int m = c->pair_mol[idx[q]], nb = c->mol_nb[m];
+            const double *fn = c->fnorm.data() + c->mol_foff[m];
+            double *Qm = c->Q.data() + c->mol_off[m];
             double qmax = 0.0, qmin = 1e300;
             for (int ia = 0; ia < na; ++ia)
                 for (int ib = 0; ib < nbb; ++ib) {
                     const int i = p.f0 + ia, j = p.f1 + ib;
                     const double raw = out[q * (size_t)nab * nab + (size_t)(ia * nbb + ib) * nab + ia * nbb + ib];
-                    const double v = c->fnorm[i] * c->fnorm[j] * std::sqrt(std::fabs(raw));
-                    c->Q[(size_t)i * nb + j] = c->Q[(size_t)j * nb + i] = v;
+                    const double v = fn[i] * fn[j] * std::sqrt(std::fabs(raw));
+                    Qm[(size_t)i * nb + j] = Qm[(size_t)j * nb + i] = v;
                     qmax = std::max(qmax, v);
                     qmin = std::min(qmin, v);
                 }
@@ -259,18 +302,22 @@ int build_schwarz(pk_ctx *c) {
     return PK_OK;
 }
 
-// Screened work lists: for every quartet class (cb >= ck) and every bra pair,
-// the ket pairs with qmax_bra * qmax_ket >= tol (and ket <= bra when cb == ck),
-// sorted by primitive count, cut into waves of at most 64 kets.  In mixed
-// precision the kets with qmax_bra * qmax_ket < cutoff form a second, FP32
-// list of the class (classes of the cooperative kernel stay FP64).
+// Screened work lists: for every quartet class (cb >= ck), every molecule and
+// every bra pair, the ket pairs of the same molecule with
+// qmax_bra * qmax_ket >= tol (and ket <= bra when cb == ck), sorted by
+// primitive count, cut into waves of at most 64 kets; the waves of all
+// molecules form one list per class.  In mixed precision the kets with
+// qmax_bra * qmax_ket < cutoff form a second, FP32 list of the class (classes
+// of the cooperative kernel stay FP64).
 int build_work(pk_ctx *c) {
     free_work(c);
     const int np = (int)c->pairs.size();
-    std::vector<std::vector<int>> by_cls(kPairClasses);
-    for (int p = 0; p < np; ++p) by_cls[c->pair_cls[p]].push_back(p);
-    for (auto &v : by_cls)
-        std::stable_sort(v.begin(), v.end(), [&](int x, int y) { return c->pair_q[x] > c->pair_q[y]; });
+    // by_cls[m][class]: pairs of molecule m, q descending
+    std::vector<std::vector<std::vector<int>>> by_cls(c->nmol, std::vector<std::vector<int>>(kPairClasses));
+    for (int p = 0; p < np; ++p) by_cls[c->pair_mol[p]][c->pair_cls[p]].push_back(p);
+    for (auto &mv : by_cls)
+        for (auto &v : mv)
+            std::stable_sort(v.begin(), v.end(), [&](int x, int y) { return c->pair_q[x] > c->pair_q[y]; });
     c->st.quartets = 0;
     c->st.primitive_quartets = 0;
     c->st.model_flops = 0.0;
@@ -279,61 +326,65 @@ int build_work(pk_ctx *c) {
         for (int ck = 0; ck <= cb; ++ck) {
             const int cls = quart_class(cb, ck);
             const ClassFns &f = kClasses[cls];
-            const std::vector<int> &B = by_cls[cb], &Kl = by_cls[ck];
-            if (B.empty() || Kl.empty()) continue;
             const bool split = mixed && f.has_tmpl;
             for (int prec : {PK_PREC_FP64, PK_PREC_MIXED}) {
                 if (prec == PK_PREC_MIXED && !split) continue;
                 ClassWork w;
                 w.cls = cls;
                 w.prec = prec;
-                std::vector<int> wb, ws, wc, kets;
+                std::vector<int> wb, ws, wc, wm, kets;
                 std::vector<int> tmp;
-                for (size_t ib = 0; ib < B.size(); ++ib) {
-                    const int b = B[ib];
-                    const double qb = c->pair_q[b];
-                    // number of kets (q descending) with qb * qk >= thr
-                    auto count = [&](double thr) {
-                        size_t lo = 0, hi = Kl.size();
-                        while (lo < hi) {
-                            const size_t mid = (lo + hi) / 2;
-                            if (qb * c->pair_q[Kl[mid]] >= thr)
-                                lo = mid + 1;
-                            else
-                                hi = mid;
+                for (int m = 0; m < c->nmol; ++m) {
+                    const std::vector<int> &B = by_cls[m][cb], &Kl = by_cls[m][ck];
+                    if (B.empty() || Kl.empty()) continue;
+                    for (size_t ib = 0; ib < B.size(); ++ib) {
+                        const int b = B[ib];
+                        const double qb = c->pair_q[b];
+                        // number of kets (q descending) with qb * qk >= thr
+                        auto count = [&](double thr) {
+                            size_t lo = 0, hi = Kl.size();
+                            while (lo < hi) {
+                                const size_t mid = (lo + hi) / 2;
+                                if (qb * c->pair_q[Kl[mid]] >= thr)
+                                    lo = mid + 1;
+                                else
+                                    hi = mid;
+                            }
+                            return cb == ck ? std::min(lo, ib + 1) : lo;
+                        };
+                        const size_t n_tol = count(c->tol);
+                        const size_t n_hi = split ? std::min(count(c->cutoff), n_tol) : n_tol;
+                        const size_t k0 = prec == PK_PREC_FP64 ? 0 : n_hi;
+                        const size_t k1 = prec == PK_PREC_FP64 ? n_hi : n_tol;
+                        const size_t n = k1 - k0;
+                        if (n == 0) continue;
+                        tmp.assign(Kl.begin() + k0, Kl.begin() + k1);
+                        std::stable_sort(tmp.begin(), tmp.end(),
+                                         [&](int x, int y) { return c->pairs[x].nprim > c->pairs[y].nprim; });
+                        const int kb = c->pairs[b].nprim;
+                        for (size_t s = 0; s < n; s += 64) {
+                            wb.push_back(b);
+                            ws.push_back((int)kets.size() + (int)s);
+                            wc.push_back((int)std::min<size_t>(64, n - s));
+                            wm.push_back(m);
                         }
-                        return cb == ck ? std::min(lo, ib + 1) : lo;
-                    };
-                    const size_t n_tol = count(c->tol);
-                    const size_t n_hi = split ? std::min(count(c->cutoff), n_tol) : n_tol;
-                    const size_t k0 = prec == PK_PREC_FP64 ? 0 : n_hi;
-                    const size_t k1 = prec == PK_PREC_FP64 ? n_hi : n_tol;
-                    const size_t n = k1 - k0;
-                    if (n == 0) continue;
-                    tmp.assign(Kl.begin() + k0, Kl.begin() + k1);
-                    std::stable_sort(tmp.begin(), tmp.end(),
-                                     [&](int x, int y) { return c->pairs[x].nprim > c->pairs[y].nprim; });
-                    const int kb = c->pairs[b].nprim;
-                    for (size_t s = 0; s < n; s += 64) {
-                        wb.push_back(b);
-                        ws.push_back((int)kets.size() + (int)s);
-                        wc.push_back((int)std::min<size_t>(64, n - s));
+                        for (size_t s = 0; s < n; ++s) {
+                            const double K = (double)kb * c->pairs[tmp[s]].nprim;
+                            w.nprimq += (int64_t)K;
+                            w.flops += pk::model_flops(f.la, f.lb, f.lc, f.ld, K);
+                        }
+                        kets.insert(kets.end(), tmp.begin(), tmp.end());
+                        w.nquart += (int64_t)n;
                     }
-                    for (size_t s = 0; s < n; ++s) {
-                        const double K = (double)kb * c->pairs[tmp[s]].nprim;
-                        w.nprimq += (int64_t)K;
-                        w.flops += pk::model_flops(f.la, f.lb, f.lc, f.ld, K);
-                    }
-                    kets.insert(kets.end(), tmp.begin(), tmp.end());
-                    w.nquart += (int64_t)n;
                 }
                 if (w.nquart == 0) continue;
                 w.nwaves = (int)wb.size();
                 std::vector<int> packed;
-                packed.reserve(3 * wb.size());
+                packed.reserve(4 * wb.size());
                 packed.insert(packed.end(), wb.begin(), wb.end());
                 packed.insert(packed.end(), ws.begin(), ws.end());
                 packed.insert(packed.end(), wc.begin(), wc.end());
+                packed.insert(packed.end(), wm.begin(), wm.end());
                 int rc;
                 if ((rc = dev_upload(&w.d_wave, packed, c->stream)) != PK_OK) return rc;
                 if ((rc = dev_upload(&w.d_kets, kets, c->stream)) != PK_OK) return rc;
@@ -345,40 +396,48 @@ int build_work(pk_ctx *c) {
         }
     // largest classes first
     std::sort(c->work.begin(), c->work.end(), [](const ClassWork &x, const ClassWork &y) { return x.flops > y.flops; });
+    // launch shapes chosen by pk_ctx_autotune for this context, if any
+    for (ClassWork &w : c->work)
+        for (const TuneEntry &t : c->tuned)
+            if (t.cls == w.cls && t.prec == w.prec) {
+                w.block = t.block;
+                w.remap = t.remap;
+            }
     PK_HIP(hipStreamSynchronize(c->stream));
     c->work_ok = true;
     return PK_OK;
 }
 
 int ensure_buffers(pk_ctx *c) {
-    const size_t n2 = (size_t)c->nb * c->nb * sizeof(double);
+    const size_t n2 = c->n2 * sizeof(double);
     if (!c->d_P) PK_HIP(hipMalloc((void **)&c->d_P, n2));
     if (!c->d_Pn) PK_HIP(hipMalloc((void **)&c->d_Pn, n2));
     if (!c->d_A) PK_HIP(hipMalloc((void **)&c->d_A, n2));
     if (!c->d_B) PK_HIP(hipMalloc((void **)&c->d_B, n2));
     if (!c->d_J) PK_HIP(hipMalloc((void **)&c->d_J, n2));
     if (!c->d_K) PK_HIP(hipMalloc((void **)&c->d_K, n2));
+    if (!c->d_G) PK_HIP(hipMalloc((void **)&c->d_G, n2));
     return PK_OK;
 }
 
-// One build: P (device) -> A, B accumulators -> J, K, G (device, any may be null)
-int build_device(pk_ctx *c, const double *dP, double *dJ, double *dK, double *dG) {
+// Enqueue the kernels of one build on the context stream: P (device) -> A, B
+// accumulators -> J, K, G (any may be null).  Class events only when profiling
+// (never while the stream is being captured).
+int launch_build(pk_ctx *c, const double *dP, double *dJ, double *dK, double *dG) {
     int rc;
-    if (!c->work_ok && (rc = build_work(c)) != PK_OK) return rc;
-    if ((rc = ensure_buffers(c)) != PK_OK) return rc;
-    const int nb = c->nb;
-    const size_t n2 = (size_t)nb * nb;
-    const int g2 = (int)((n2 + 255) / 256);
     hipStream_t s = c->stream;
-    PK_HIP(hipEventRecord(c->ev0, s));
-    hipLaunchKernelGGL(k_scale_density, dim3(g2), dim3(256), 0, s, dP, c->d_fnorm, c->d_Pn, nb);
-    hipLaunchKernelGGL(k_fill, dim3(g2), dim3(256), 0, s, c->d_A, n2, 0.0);
-    hipLaunchKernelGGL(k_fill, dim3(g2), dim3(256), 0, s, c->d_B, n2, 0.0);
+    const MolTab t{c->d_mol_off, c->d_mol_nb, c->d_mol_foff};
+    const dim3 g2((unsigned)(((size_t)c->nbmax * c->nbmax + 255) / 256), (unsigned)c->nmol);
+    const int gf = (int)((c->n2 + 255) / 256);
+    hipLaunchKernelGGL(k_scale_density, g2, dim3(256), 0, s, dP, c->d_fnorm, c->d_Pn, t);
+    hipLaunchKernelGGL(k_fill, dim3(gf), dim3(256), 0, s, c->d_A, c->n2, 0.0);
+    hipLaunchKernelGGL(k_fill, dim3(gf), dim3(256), 0, s, c->d_B, c->n2, 0.0);
     int nl = 3;
     for (ClassWork &w : c->work) {
         const int *wv = w.d_wave;
-        pk::JKArgs a{wv, wv + w.nwaves, wv + 2 * w.nwaves, w.d_kets, c->d_pairs, c->d_prims,
-                     c->d_Pn, c->d_Q, c->d_A, c->d_B, c->tol, nb, w.nwaves};
+        pk::JKArgs a{wv, wv + w.nwaves, wv + 2 * w.nwaves, wv + 3 * w.nwaves, w.d_kets, c->d_pairs,
+                     c->d_prims, c->d_mol_off, c->d_mol_nb, c->d_Pn, c->d_Q, c->d_A, c->d_B,
+                     c->tol, w.nwaves, w.remap};
         if (c->profile) {
             if (!w.e0) PK_HIP(hipEventCreate(&w.e0));
             if (!w.e1) PK_HIP(hipEventCreate(&w.e1));
@@ -387,16 +446,22 @@ int build_device(pk_ctx *c, const double *dP, double *dJ, double *dK, double *dG
         const int before = nl;
         const ClassFns &f = kClasses[w.cls];
         const bool fp32 = w.prec == PK_PREC_MIXED;
+        const int wpb = w.block / 64;
+        int grid = (w.nwaves + wpb - 1) / wpb;
+        if (w.remap) grid = (grid + 7) / 8 * 8;
         if (use_os(c, f)) {
-            (fp32 ? f.jk_os32 : f.jk_os)(a, (w.nwaves + 3) / 4, s, nl);
+            (fp32 ? f.jk_os32 : f.jk_os)(a, grid, w.block, s, nl);
         } else if (f.has_tmpl) {
-            (fp32 ? f.jk_rys32 : f.jk_rys)(a, (w.nwaves + 3) / 4, s, nl);
+            (fp32 ? f.jk_rys32 : f.jk_rys)(a, grid, w.block, s, nl);
         } else {
             pk::GenArgs g{};
             g.wave_bra = a.wave_bra;
             g.wave_start = a.wave_start;
             g.wave_count = a.wave_count;
+            g.wave_mol = a.wave_mol;
             g.kets = a.kets;
+            g.mol_off = a.mol_off;
+            g.mol_nb = a.mol_nb;
             g.pairs = a.pairs;
             g.prims = a.prims;
             g.P = a.P;
@@ -404,7 +469,6 @@ int build_device(pk_ctx *c, const double *dP, double *dJ, double *dK, double *dG
             g.A = a.A;
             g.B = a.B;
             g.tol = a.tol;
-            g.nb = nb;
             g.nitems = w.nwaves;
             g.la = f.la;
             g.lb = f.lb;
@@ -416,11 +480,51 @@ int build_device(pk_ctx *c, const double *dP, double *dJ, double *dK, double *dG
         w.kernels = nl - before;
         if (c->profile) PK_HIP(hipEventRecord(w.e1, s));
     }
-    hipLaunchKernelGGL(k_finalize, dim3(g2), dim3(256), 0, s, c->d_A, c->d_B, c->d_fnorm, dJ, dK, dG, nb);
+    hipLaunchKernelGGL(k_finalize, g2, dim3(256), 0, s, c->d_A, c->d_B, c->d_fnorm, dJ, dK, dG, t);
     ++nl;
     PK_HIP(hipGetLastError());
-    PK_HIP(hipEventRecord(c->ev1, s));
     c->st.kernel_launches = nl;
+    return PK_OK;
+}
+
+// One build.  With graphs enabled (and profiling off) the kernels of a build
+// are captured once into a hipGraph that reads the context's own density
+// buffer and writes its own J, K, G buffers, and every later build replays
+// it; the caller's buffers are copied in and out around the replay.
+int build_device(pk_ctx *c, const double *dP, double *dJ, double *dK, double *dG) {
+    int rc;
+    if (!c->work_ok && (rc = build_work(c)) != PK_OK) return rc;
+    if ((rc = ensure_buffers(c)) != PK_OK) return rc;
+    hipStream_t s = c->stream;
+    const size_t bytes = c->n2 * sizeof(double);
+    if (!c->graph_on || c->profile) {
+        PK_HIP(hipEventRecord(c->ev0, s));
+        if ((rc = launch_build(c, dP, dJ, dK, dG)) != PK_OK) return rc;
+        PK_HIP(hipEventRecord(c->ev1, s));
+        return PK_OK;
+    }
+    if (!c->graph_ok) {
+        drop_graph(c);
+        PK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        const int rc_cap = launch_build(c, c->d_P, c->d_J, c->d_K, c->d_G);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(s, &g);
+        if (rc_cap != PK_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc_cap;
+        }
+        PK_HIP(e);
+        c->graph = g;
+        PK_HIP(hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0));
+        c->graph_ok = true;
+    }
+    PK_HIP(hipEventRecord(c->ev0, s));
+    if (dP != c->d_P) PK_HIP(hipMemcpyAsync(c->d_P, dP, bytes, hipMemcpyDeviceToDevice, s));
+    PK_HIP(hipGraphLaunch(c->gexec, s));
+    if (dJ && dJ != c->d_J) PK_HIP(hipMemcpyAsync(dJ, c->d_J, bytes, hipMemcpyDeviceToDevice, s));
+    if (dK && dK != c->d_K) PK_HIP(hipMemcpyAsync(dK, c->d_K, bytes, hipMemcpyDeviceToDevice, s));
+    if (dG && dG != c->d_G) PK_HIP(hipMemcpyAsync(dG, c->d_G, bytes, hipMemcpyDeviceToDevice, s));
+    PK_HIP(hipEventRecord(c->ev1, s));
     return PK_OK;
 }
 
@@ -451,37 +555,33 @@ extern "C" {
 
 const char *pk_last_error(void) { return g_err.c_str(); }
 
-int pk_ctx_create(const pk_basis_desc *d, pk_ctx **out) {
-    if (!d || !out || d->nshells <= 0 || !d->center || !d->l || !d->nprim || !d->exponent || !d->coef || !d->prim_norm)
-        return fail(PK_ERR_ARG, "pk_ctx_create: null or empty basis description");
-    *out = nullptr;
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
-        return fail(PK_ERR_HIP, "pk_ctx_create: no HIP device visible");
-    pk_ctx *c = new pk_ctx;
-    c->ns = d->nshells;
-    int off = 0, f0 = 0;
-    for (int s = 0; s < c->ns; ++s) {
+}  // extern "C"
+
+namespace {
+
+// Shells, component norms and shell pairs of molecule m (pairs within the
+// molecule only), appended to the context's arrays.
+int add_molecule(pk_ctx *c, const pk_basis_desc *d, int m) {
+    if (!d || d->nshells <= 0 || !d->center || !d->l || !d->nprim || !d->exponent || !d->coef || !d->prim_norm)
+        return fail(PK_ERR_ARG, "pk_ctx_create: null or empty basis description (molecule " + std::to_string(m) + ")");
+    const int s0 = c->ns;
+    int off = (int)c->ex.size(), f0 = 0;
+    const int poff0 = off;
+    for (int s = 0; s < d->nshells; ++s) {
         const int l = d->l[s], np = d->nprim[s];
-        if (l < 0 || l > kMaxL) {
-            delete c;
-            return fail(PK_ERR_UNSUPPORTED, "pk_ctx_create: shell " + std::to_string(s) + " has l=" +
-                                                std::to_string(l) + "; this build supports l <= " +
-                                                std::to_string(kMaxL));
-        }
-        if (np <= 0) {
-            delete c;
-            return fail(PK_ERR_ARG, "pk_ctx_create: shell with no primitives");
-        }
+        if (l < 0 || l > kMaxL)
+            return fail(PK_ERR_UNSUPPORTED, "pk_ctx_create: shell " + std::to_string(s) + " has l=" + std::to_string(l) +
+                                                "; this build supports l <= " + std::to_string(kMaxL));
+        if (np <= 0) return fail(PK_ERR_ARG, "pk_ctx_create: shell with no primitives");
         c->sh_l.push_back(l);
         c->sh_np.push_back(np);
         c->sh_poff.push_back(off);
         c->sh_f0.push_back(f0);
         for (int q = 0; q < 3; ++q) c->sh_c.push_back(d->center[3 * s + q]);
         for (int k = 0; k < np; ++k) {
-            c->ex.push_back(d->exponent[off + k]);
-            c->co.push_back(d->coef[off + k]);
-            c->pn.push_back(d->prim_norm[off + k]);
+            c->ex.push_back(d->exponent[off - poff0 + k]);
+            c->co.push_back(d->coef[off - poff0 + k]);
+            c->pn.push_back(d->prim_norm[off - poff0 + k]);
         }
         // component norms 1/sqrt((2lx-1)!!(2ly-1)!!(2lz-1)!!)  (gaussian.cpp:286-298)
         for (int k = 0; k < pk::ncart(l); ++k) {
@@ -499,25 +599,31 @@ int pk_ctx_create(const pk_basis_desc *d, pk_ctx **out) {
         off += np;
         f0 += pk::ncart(l);
     }
-    c->nb = f0;
+    c->ns += d->nshells;
+    c->mol_foff.push_back(c->nb);
+    c->mol_nb.push_back(f0);
+    c->mol_off.push_back((long long)c->n2);
+    c->nb += f0;
+    c->n2 += (size_t)f0 * f0;
+    c->nbmax = std::max(c->nbmax, f0);
     // shell pairs a >= b, first shell carries the larger l (types.h:606-675 pair data)
-    for (int A = 0; A < c->ns; ++A)
-        for (int Bs = 0; Bs <= A; ++Bs) {
-            const int s0 = c->sh_l[A] >= c->sh_l[Bs] ? A : Bs, s1 = s0 == A ? Bs : A;
+    for (int A = s0; A < c->ns; ++A)
+        for (int Bs = s0; Bs <= A; ++Bs) {
+            const int sa = c->sh_l[A] >= c->sh_l[Bs] ? A : Bs, sb = sa == A ? Bs : A;
             pk::PairDev p{};
-            p.f0 = c->sh_f0[s0];
-            p.f1 = c->sh_f0[s1];
+            p.f0 = c->sh_f0[sa];
+            p.f1 = c->sh_f0[sb];
             p.diag = A == Bs;
             p.prim_off = (int)c->prims.size();
-            p.nprim = c->sh_np[s0] * c->sh_np[s1];
-            const double *Ac = &c->sh_c[3 * s0], *Bc = &c->sh_c[3 * s1];
+            p.nprim = c->sh_np[sa] * c->sh_np[sb];
+            const double *Ac = &c->sh_c[3 * sa], *Bc = &c->sh_c[3 * sb];
             for (int q = 0; q < 3; ++q) p.ab[q] = Ac[q] - Bc[q];
             const double R2 = p.ab[0] * p.ab[0] + p.ab[1] * p.ab[1] + p.ab[2] * p.ab[2];
-            for (int i = 0; i < c->sh_np[s0]; ++i)
-                for (int j = 0; j < c->sh_np[s1]; ++j) {
-                    const double al = c->ex[c->sh_poff[s0] + i], be = c->ex[c->sh_poff[s1] + j];
-                    const double ca = c->co[c->sh_poff[s0] + i] * c->pn[c->sh_poff[s0] + i];
-                    const double cb = c->co[c->sh_poff[s1] + j] * c->pn[c->sh_poff[s1] + j];
+            for (int i = 0; i < c->sh_np[sa]; ++i)
+                for (int j = 0; j < c->sh_np[sb]; ++j) {
+                    const double al = c->ex[c->sh_poff[sa] + i], be = c->ex[c->sh_poff[sb] + j];
+                    const double ca = c->co[c->sh_poff[sa] + i] * c->pn[c->sh_poff[sa] + i];
+                    const double cb = c->co[c->sh_poff[sb] + j] * c->pn[c->sh_poff[sb] + j];
                     pk::PrimDev pp{};
                     pp.zeta = al + be;
                     const double iz = 1.0 / pp.zeta;
@@ -532,9 +638,32 @@ int pk_ctx_create(const pk_basis_desc *d, pk_ctx **out) {
                     c->prims.push_back(pp);
                 }
             c->pairs.push_back(p);
-            c->pair_cls.push_back(pk::pair_class(c->sh_l[s0], c->sh_l[s1]));
+            c->pair_cls.push_back(pk::pair_class(c->sh_l[sa], c->sh_l[sb]));
             c->pair_q.push_back(0.0);
+            c->pair_mol.push_back(m);
         }
+    return PK_OK;
+}
+
+int ctx_build(const pk_basis_desc *descs, int nmol, pk_ctx **out) {
+    if (!descs || !out || nmol <= 0) return fail(PK_ERR_ARG, "pk_ctx_create: null or empty basis description");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(PK_ERR_HIP, "pk_ctx_create: no HIP device visible");
+    pk_ctx *c = new pk_ctx;
+    c->nmol = nmol;
+    for (int m = 0; m < nmol; ++m) {
+        const int rc = add_molecule(c, &descs[m], m);
+        if (rc != PK_OK) {
+            delete c;
+            return rc;
+        }
+    }
+    if (c->prims.size() > (size_t)INT32_MAX || c->n2 > (size_t)1 << 40) {
+        delete c;
+        return fail(PK_ERR_ARG, "pk_ctx_create: batch too large");
+    }
     c->st.nbasis = c->nb;
     c->st.nshells = c->ns;
     c->st.shell_pairs = (int64_t)c->pairs.size();
@@ -545,7 +674,10 @@ int pk_ctx_create(const pk_basis_desc *d, pk_ctx **out) {
         return fail(PK_ERR_HIP, "pk_ctx_create: stream/event creation failed");
     }
     c->own_stream = true;
-    if ((rc = dev_upload(&c->d_pairs, c->pairs, c->stream)) != PK_OK ||
+    if ((rc = dev_upload(&c->d_mol_off, c->mol_off, c->stream)) != PK_OK ||
+        (rc = dev_upload(&c->d_mol_nb, c->mol_nb, c->stream)) != PK_OK ||
+        (rc = dev_upload(&c->d_mol_foff, c->mol_foff, c->stream)) != PK_OK ||
+        (rc = dev_upload(&c->d_pairs, c->pairs, c->stream)) != PK_OK ||
         (rc = dev_upload(&c->d_prims, c->prims, c->stream)) != PK_OK ||
         (rc = dev_upload(&c->d_fnorm, c->fnorm, c->stream)) != PK_OK || (rc = build_schwarz(c)) != PK_OK ||
         (rc = dev_upload(&c->d_Q, c->Q, c->stream)) != PK_OK ||
@@ -564,12 +696,99 @@ int pk_ctx_create(const pk_basis_desc *d, pk_ctx **out) {
     return PK_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
+int pk_ctx_create(const pk_basis_desc *d, pk_ctx **out) { return ctx_build(d, 1, out); }
+
+int pk_batch_create(const pk_basis_desc *descs, int nmol, pk_ctx **out) { return ctx_build(descs, nmol, out); }
+
+int pk_ctx_nmol(const pk_ctx *c) { return c ? c->nmol : -1; }
+
+int pk_ctx_mol_layout(const pk_ctx *c, int32_t *nb, int64_t *offset) {
+    if (!c) return fail(PK_ERR_ARG, "pk_ctx_mol_layout: null context");
+    for (int m = 0; m < c->nmol; ++m) {
+        if (nb) nb[m] = c->mol_nb[m];
+        if (offset) offset[m] = c->mol_off[m];
+    }
+    return PK_OK;
+}
+
+// Launch-shape autotuner: every class list is timed (HIP events, `reps` builds
+// per candidate after one untimed build) under each candidate shape --
+// workgroups of 1, 2 or 4 waves, with and without XCD-contiguous wave ranges
+// -- and keeps the fastest; the choice persists for the context and is
+// re-applied when the work lists are rebuilt.
+int pk_ctx_autotune(pk_ctx *c, int reps) {
+    if (!c) return fail(PK_ERR_ARG, "pk_ctx_autotune: null context");
+    if (reps < 1) reps = 1;
+    int rc;
+    if (!c->work_ok && (rc = build_work(c)) != PK_OK) return rc;
+    if ((rc = ensure_buffers(c)) != PK_OK) return rc;
+    hipStream_t s = c->stream;
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)((c->n2 + 255) / 256)), dim3(256), 0, s, c->d_P, c->n2, 0.01);
+    const bool prof = c->profile;
+    c->profile = true;
+    const int cand[6][2] = {{64, 0}, {128, 0}, {256, 0}, {64, 1}, {128, 1}, {256, 1}};
+    const size_t nw = c->work.size();
+    std::vector<double> best(nw, 1e300);
+    std::vector<int> pick(nw, 2);
+    auto run = [&](int k, int nrep, bool keep) -> int {
+        for (ClassWork &w : c->work) {
+            w.block = cand[k][0];
+            w.remap = cand[k][1];
+        }
+        std::vector<double> t(nw, 0.0);
+        for (int r = 0; r < nrep; ++r) {
+            int rc2;
+            if ((rc2 = launch_build(c, c->d_P, c->d_J, c->d_K, nullptr)) != PK_OK) return rc2;
+            PK_HIP(hipStreamSynchronize(s));
+            for (size_t i = 0; i < nw; ++i) {
+                float ms = 0.f;
+                PK_HIP(hipEventElapsedTime(&ms, c->work[i].e0, c->work[i].e1));
+                t[i] += ms;
+            }
+        }
+        if (keep)
+            for (size_t i = 0; i < nw; ++i)
+                if (t[i] < best[i]) {
+                    best[i] = t[i];
+                    pick[i] = k;
+                }
+        return PK_OK;
+    };
+    rc = run(2, 1, false);
+    for (int k = 0; k < 6 && rc == PK_OK; ++k) rc = run(k, reps, true);
+    c->profile = prof;
+    if (rc != PK_OK) return rc;
+    c->tuned.clear();
+    for (size_t i = 0; i < nw; ++i) {
+        ClassWork &w = c->work[i];
+        w.block = cand[pick[i]][0];
+        w.remap = cand[pick[i]][1];
+        w.ms_total = 0.0;
+        w.builds = 0;
+        c->tuned.push_back(TuneEntry{w.cls, w.prec, w.block, w.remap});
+    }
+    drop_graph(c);
+    return PK_OK;
+}
+
+int pk_ctx_set_graph(pk_ctx *c, int enable) {
+    if (!c) return fail(PK_ERR_ARG, "pk_ctx_set_graph: null context");
+    c->graph_on = enable != 0;
+    if (!c->graph_on) drop_graph(c);
+    return PK_OK;
+}
+
 int pk_ctx_destroy(pk_ctx *c) {
     if (!c) return PK_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_work(c);
     for (void *p : {(void *)c->d_pairs, (void *)c->d_prims, (void *)c->d_Q, (void *)c->d_fnorm, (void *)c->d_P,
-                    (void *)c->d_Pn, (void *)c->d_A, (void *)c->d_B, (void *)c->d_J, (void *)c->d_K})
+                    (void *)c->d_Pn, (void *)c->d_A, (void *)c->d_B, (void *)c->d_J, (void *)c->d_K, (void *)c->d_G,
+                    (void *)c->d_mol_off, (void *)c->d_mol_nb, (void *)c->d_mol_foff})
         if (p) (void)hipFree(p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -641,6 +860,8 @@ int pk_ctx_class_stats(const pk_ctx *c, int i, pk_class_stats *o) {
     o->ms_total = w.ms_total;
     o->builds = w.builds;
     o->precision = w.prec;
+    o->block = w.block;
+    o->remap = w.remap;
     return PK_OK;
 }
 
@@ -655,17 +876,17 @@ static int fock_common(pk_ctx *c, const double *P, double *J, double *K, double 
     if (memkind != PK_MEM_HOST && memkind != PK_MEM_DEVICE) return fail(PK_ERR_ARG, "pk_fock: bad memkind");
     int rc;
     if ((rc = ensure_buffers(c)) != PK_OK) return rc;
-    const size_t n2 = (size_t)c->nb * c->nb * sizeof(double);
+    const size_t n2 = c->n2 * sizeof(double);
     if (memkind == PK_MEM_DEVICE) {
         if ((rc = build_device(c, P, J, K, G)) != PK_OK) return rc;
         return finish_timing(c);
     }
     PK_HIP(hipMemcpyAsync(c->d_P, P, n2, hipMemcpyHostToDevice, c->stream));
-    double *dJ = J ? c->d_J : nullptr, *dK = K ? c->d_K : nullptr, *dG = G ? c->d_J : nullptr;
+    double *dJ = J ? c->d_J : nullptr, *dK = K ? c->d_K : nullptr, *dG = G ? c->d_G : nullptr;
     if ((rc = build_device(c, c->d_P, dJ, dK, dG)) != PK_OK) return rc;
     if (J) PK_HIP(hipMemcpyAsync(J, c->d_J, n2, hipMemcpyDeviceToHost, c->stream));
     if (K) PK_HIP(hipMemcpyAsync(K, c->d_K, n2, hipMemcpyDeviceToHost, c->stream));
-    if (G) PK_HIP(hipMemcpyAsync(G, c->d_J, n2, hipMemcpyDeviceToHost, c->stream));
+    if (G) PK_HIP(hipMemcpyAsync(G, c->d_G, n2, hipMemcpyDeviceToHost, c->stream));
     PK_HIP(hipStreamSynchronize(c->stream));
     return finish_timing(c);
 }
@@ -682,7 +903,7 @@ int pk_fock_2e(pk_ctx *c, const double *P, double *G, int memkind) {
 int pk_fock_jk_uhf(pk_ctx *c, const double *Pa, const double *Pb, double *Jt, double *Ka, double *Kb, int memkind) {
     if (!c || !Pa || !Pb) return fail(PK_ERR_ARG, "pk_fock_jk_uhf: null argument");
     if (memkind != PK_MEM_HOST) return fail(PK_ERR_UNSUPPORTED, "pk_fock_jk_uhf: host pointers only in this build");
-    const size_t n2 = (size_t)c->nb * c->nb;
+    const size_t n2 = c->n2;
     std::vector<double> Ja(n2), Jb(n2), ka(n2), kb(n2);
     int rc;
     if ((rc = pk_fock_jk(c, Pa, Ja.data(), ka.data(), PK_MEM_HOST)) != PK_OK) return rc;
@@ -697,7 +918,7 @@ int pk_fock_jk_uhf(pk_ctx *c, const double *Pa, const double *Pb, double *Jt, do
 
 int pk_schwarz(pk_ctx *c, double *Q, int memkind) {
     if (!c || !Q) return fail(PK_ERR_ARG, "pk_schwarz: null argument");
-    const size_t n2 = (size_t)c->nb * c->nb * sizeof(double);
+    const size_t n2 = c->n2 * sizeof(double);
     if (memkind == PK_MEM_DEVICE) {
         PK_HIP(hipMemcpyAsync(Q, c->d_Q, n2, hipMemcpyDeviceToDevice, c->stream));
         PK_HIP(hipStreamSynchronize(c->stream));
@@ -709,6 +930,7 @@ int pk_schwarz(pk_ctx *c, double *Q, int memkind) {
 
 int pk_eri_tensor(pk_ctx *c, double *out) {
     if (!c || !out) return fail(PK_ERR_ARG, "pk_eri_tensor: null argument");
+    if (c->nmol != 1) return fail(PK_ERR_ARG, "pk_eri_tensor: one molecule only");
     const size_t nb = c->nb, nb2 = nb * nb, nb3 = nb2 * nb;
     if ((double)nb3 * nb * 8.0 > 2147483648.0) return fail(PK_ERR_ARG, "pk_eri_tensor: nb^4 tensor exceeds 2 GiB");
     std::fill(out, out + nb3 * nb, 0.0);

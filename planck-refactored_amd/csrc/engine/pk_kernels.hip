@@ -6,17 +6,17 @@
     PK_DEFINE_IF_##shard(a, b, c, d)
 
 #define PK_DEFINE_NOW(a, b, c, d)                                                              \
-    void pk_launch_jk_rys_##a##b##c##d(const pk::JKArgs &x, int g, hipStream_t s, int &n) {   \
-        pk::launch_jk_rys<double, a, b, c, d, 0>(x, g, s, n);                                  \
+    void pk_launch_jk_rys_##a##b##c##d(const pk::JKArgs &x, int g, int b_, hipStream_t s, int &n) {   \
+        pk::launch_jk_rys<double, a, b, c, d, 0>(x, g, b_, s, n);                                  \
     }                                                                                          \
-    void pk_launch_jk_os_##a##b##c##d(const pk::JKArgs &x, int g, hipStream_t s, int &n) {    \
-        pk::launch_jk_os<double, a, b, c, d>(x, g, s, n);                                      \
+    void pk_launch_jk_os_##a##b##c##d(const pk::JKArgs &x, int g, int b_, hipStream_t s, int &n) {    \
+        pk::launch_jk_os<double, a, b, c, d>(x, g, b_, s, n);                                      \
     }                                                                                          \
-    void pk_launch_jk_rys32_##a##b##c##d(const pk::JKArgs &x, int g, hipStream_t s, int &n) { \
-        pk::launch_jk_rys<float, a, b, c, d, 0>(x, g, s, n);                                   \
+    void pk_launch_jk_rys32_##a##b##c##d(const pk::JKArgs &x, int g, int b_, hipStream_t s, int &n) { \
+        pk::launch_jk_rys<float, a, b, c, d, 0>(x, g, b_, s, n);                                   \
     }                                                                                          \
-    void pk_launch_jk_os32_##a##b##c##d(const pk::JKArgs &x, int g, hipStream_t s, int &n) {  \
-        pk::launch_jk_os<float, a, b, c, d>(x, g, s, n);                                       \
+    void pk_launch_jk_os32_##a##b##c##d(const pk::JKArgs &x, int g, int b_, hipStream_t s, int &n) {  \
+        pk::launch_jk_os<float, a, b, c, d>(x, g, b_, s, n);                                       \
     }
 
 #define PK_SKIP(a, b, c, d)

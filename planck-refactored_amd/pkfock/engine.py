@@ -42,7 +42,7 @@ class ClassStats(ctypes.Structure):
     _fields_ = [("la", ctypes.c_int32), ("lb", ctypes.c_int32), ("lc", ctypes.c_int32), ("ld", ctypes.c_int32),
                 ("kernels", ctypes.c_int32), ("quartets", ctypes.c_int64), ("primitive_quartets", ctypes.c_int64),
                 ("model_flops", ctypes.c_double), ("ms_total", ctypes.c_double), ("builds", ctypes.c_int64),
-                ("precision", ctypes.c_int32)]
+                ("precision", ctypes.c_int32), ("block", ctypes.c_int32), ("remap", ctypes.c_int32)]
 
 
 _lib = None
@@ -70,6 +70,11 @@ def load():
     lib.pk_ctx_profile.argtypes = [vp, ip]
     lib.pk_ctx_class_count.argtypes = [vp]
     lib.pk_ctx_class_stats.argtypes = [vp, ip, ctypes.POINTER(ClassStats)]
+    lib.pk_batch_create.argtypes = [ctypes.POINTER(BasisDesc), ip, ctypes.POINTER(vp)]
+    lib.pk_ctx_nmol.argtypes = [vp]
+    lib.pk_ctx_mol_layout.argtypes = [vp, vp, vp]
+    lib.pk_ctx_set_graph.argtypes = [vp, ip]
+    lib.pk_ctx_autotune.argtypes = [vp, ip]
     lib.pk_last_error.restype = ctypes.c_char_p
     _lib = lib
     return lib
@@ -115,15 +120,51 @@ class Shells:
 
 
 class FockEngine:
-    """One engine context per geometry (pk_ctx)."""
+    """One engine context per geometry (pk_ctx).  A list of Shells makes a
+    batch context (pk_batch_create): matrices are then packed, molecule m's
+    block at element offsets[m] with side nbs[m] (see pack / unpack)."""
 
     def __init__(self, shells, engine="os", tol_eri=1e-10):
         lib = load()
         self._keep = shells
         self.ctx = ctypes.c_void_p()
-        _check(lib.pk_ctx_create(ctypes.byref(shells.desc()), ctypes.byref(self.ctx)), "pk_ctx_create")
+        if isinstance(shells, (list, tuple)):
+            descs = (BasisDesc * len(shells))(*[s.desc() for s in shells])
+            _check(lib.pk_batch_create(descs, len(shells), ctypes.byref(self.ctx)), "pk_batch_create")
+        else:
+            _check(lib.pk_ctx_create(ctypes.byref(shells.desc()), ctypes.byref(self.ctx)), "pk_ctx_create")
         self.nbasis = lib.pk_ctx_nbasis(self.ctx)
+        self.nmol = lib.pk_ctx_nmol(self.ctx)
+        nbs = np.zeros(self.nmol, dtype=np.int32)
+        offs = np.zeros(self.nmol, dtype=np.int64)
+        _check(lib.pk_ctx_mol_layout(self.ctx, _ptr(nbs), _ptr(offs)), "pk_ctx_mol_layout")
+        self.nbs, self.offsets = nbs, offs
+        self.packed_size = int(offs[-1] + nbs[-1] ** 2)
         self.set(engine, tol_eri)
+
+    def pack(self, mats):
+        """List of nb_m x nb_m matrices -> packed vector."""
+        out = np.empty(self.packed_size)
+        for m, a in enumerate(mats):
+            n = int(self.nbs[m])
+            out[self.offsets[m]:self.offsets[m] + n * n] = np.ascontiguousarray(a, dtype=np.float64).ravel()
+        return out
+
+    def unpack(self, vec):
+        return [vec[o:o + int(n) * int(n)].reshape(int(n), int(n)) for o, n in zip(self.offsets, self.nbs)]
+
+    def jk_batch(self, Ps):
+        """J, K of every molecule of a batch context (lists of matrices)."""
+        P = self.pack(Ps)
+        J, K = np.zeros_like(P), np.zeros_like(P)
+        _check(load().pk_fock_jk(self.ctx, _ptr(P), _ptr(J), _ptr(K), MEM_HOST), "pk_fock_jk")
+        return self.unpack(J), self.unpack(K)
+
+    def set_graph(self, enable=True):
+        _check(load().pk_ctx_set_graph(self.ctx, int(bool(enable))), "pk_ctx_set_graph")
+
+    def autotune(self, reps=2):
+        _check(load().pk_ctx_autotune(self.ctx, int(reps)), "pk_ctx_autotune")
 
     def set(self, engine="os", tol_eri=1e-10, precision=PREC_FP64, fp32_cutoff=0.0):
         e = ENGINE[engine] if isinstance(engine, str) else int(engine)
@@ -157,9 +198,9 @@ class FockEngine:
         return out
 
     def schwarz(self):
-        Q = np.zeros((self.nbasis, self.nbasis))
+        Q = np.zeros(self.packed_size)
         _check(load().pk_schwarz(self.ctx, _ptr(Q), MEM_HOST), "pk_schwarz")
-        return Q
+        return Q.reshape(self.nbasis, self.nbasis) if self.nmol == 1 else self.unpack(Q)
 
     def eri_tensor(self):
         n = self.nbasis
