@@ -6,11 +6,11 @@
 // and every ERI-output request (Schwarz table, ERI tensor).
 //
 // Per primitive quartet: lanes r < n evaluate the Rys roots; lanes t < 3n build
-// the 2-D integrals of root t/3 and axis t%3 in registers and write the fully
+// the 2-D integrals of root t/3 and axis t%3 into LDS; all lanes then form the
 // transferred 1-D factors G[a][b][c][d] (closed-form transfers
 //   (a|b) = sum_k C(b,k) (A-B)^(b-k) (a+k|0),  (c|d) likewise with C-D)
-// to LDS; then every lane adds sum_r Gx Gy Gz for its share of the (ab|cd)
-// components into LDS accumulators.  J/K digestion reduces each output element
+// and add sum_r Gx Gy Gz for their share of the (ab|cd) components into LDS
+// accumulators.  J/K digestion reduces each output element
 // in one lane (no atomic contention inside the quartet), then FP64 atomics.
 // Reference semantics as in eri_prim.h (rys.cpp:250-485, os.cpp:837-879).
 #pragma once
@@ -19,11 +19,8 @@
 
 namespace pk {
 
-constexpr int GEN_MAX_L = 3;                       // per shell
-constexpr int GEN_NR = 7;                          // roots for L <= 12
-constexpr int GEN_D = GEN_MAX_L + 1;               // per-axis a, b, c, d extent
-constexpr int GEN_G = GEN_D * GEN_D * GEN_D * GEN_D;  // G entries per (root, axis)
-constexpr int GEN_MAXC = 10 * 10 * 10 * 10;        // (ff|ff) components
+constexpr int GEN_MAX_L = 3;  // per shell
+constexpr int GEN_NR = 7;     // roots for L <= 12; also the 2-D extent E + 1 <= 7
 
 struct GenArgs {
     // JK mode: wave lists of one class (as JKArgs); ERI mode: explicit quartets
@@ -49,15 +46,6 @@ struct GenArgs {
     int eri_mode;
 };
 
-__device__ __forceinline__ int comp_axis(int l, int k, int q) {
-    int lx = l;
-    while (k > l - lx) {
-        k -= l - lx + 1;
-        --lx;
-    }
-    return q == 0 ? lx : (q == 1 ? l - lx - k : k);
-}
-
 // C(n, k) for 0 <= k <= n <= 3
 __device__ __forceinline__ double binom(int n, int k) {
     return (n == 3 && (k == 1 || k == 2)) ? 3.0 : ((n == 2 && k == 1) ? 2.0 : 1.0);
@@ -82,6 +70,34 @@ __device__ __forceinline__ void rys_root_rt(int n, double T, int r, double &u, d
     }
 }
 
+// Sums of nterm terms for nout outputs over the 64 lanes of the workgroup:
+// lpo lanes (a power of two) per output each add a strided share, then a
+// butterfly inside the aligned lane group; the group's first lane emits.
+// Every lane calls it (the shuffles need the whole wave).
+template <typename Term, typename Emit>
+__device__ __forceinline__ void seg_reduce(int lane, int nout, int nterm, Term term, Emit emit) {
+    int lpo = 1;
+    while (lpo < 64 && lpo * 2 * nout <= 64) lpo *= 2;
+    const int opp = 64 / lpo;
+    const int sl = lane & (lpo - 1);
+    for (int o0 = 0; o0 < nout; o0 += opp) {
+        const int o = o0 + lane / lpo;
+        double s = 0.0;
+        if (o < nout)
+            for (int t = sl; t < nterm; t += lpo) s += term(o, t);
+        for (int off = lpo >> 1; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
+        if (o < nout && sl == 0) emit(o, s);
+    }
+}
+
+// Cartesian exponents (x, y, z) of component k of a shell of angular momentum
+// l <= 3, in the order lx descending, then ly descending
+static __device__ const unsigned char kCartTab[4][10][3] = {
+    {{0, 0, 0}},
+    {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}},
+    {{2, 0, 0}, {1, 1, 0}, {1, 0, 1}, {0, 2, 0}, {0, 1, 1}, {0, 0, 2}},
+    {{3, 0, 0}, {2, 1, 0}, {2, 0, 1}, {1, 2, 0}, {1, 1, 1}, {1, 0, 2}, {0, 3, 0}, {0, 2, 1}, {0, 1, 2}, {0, 0, 3}}};
+
 __global__ __launch_bounds__(64) void k_generic(GenArgs a) {
     extern __shared__ double smem[];
     __shared__ double su[GEN_NR], sw[GEN_NR];
@@ -93,8 +109,12 @@ __global__ __launch_bounds__(64) void k_generic(GenArgs a) {
     const int NC = (lc + 1) * (lc + 2) / 2, ND = (ld + 1) * (ld + 2) / 2;
     const int NCD = NC * ND, NABCD = NA * NB * NCD;
     const int E = la + lb, F = lc + ld, n = (E + F) / 2 + 1;
-    double *G = smem;               // [n][3][GEN_G]
-    double *acc = G + n * 3 * GEN_G;  // [NABCD]
+    const int F1 = F + 1, NI = (E + 1) * F1;
+    const int D2 = lb + 1, D3 = lc + 1, D4 = ld + 1;
+    const int NG = (la + 1) * D2 * D3 * D4;  // transferred 1-D factors per (root, axis)
+    double *I2 = smem;                       // [n * 3][E + 1][F + 1]
+    double *G = I2 + n * 3 * NI;             // [n * 3][la + 1][lb + 1][lc + 1][ld + 1]
+    double *acc = G + n * 3 * NG;            // [NABCD]
 
     int bra, kbeg, kcnt, nb = 0;
     const double *P = nullptr, *Qm = nullptr;
@@ -139,8 +159,10 @@ __global__ __launch_bounds__(64) void k_generic(GenArgs a) {
                     sw[lane] = w;
                 }
                 __syncthreads();
+                // 2-D integrals I[e][f] of root r, axis ax (lane 3r + ax), one
+                // column f at a time with the previous two columns in registers
                 if (lane < 3 * n) {
-                    const int r = lane / 3, ax = lane % 3;
+                    const int r = lane / 3, ax = lane - 3 * r;
                     const double ur = su[r];
                     const double B00 = 0.5 * inv_de * ur;
                     const double B10 = (0.5 / p.zeta) * (1.0 - rho / p.zeta * ur);
@@ -148,58 +170,63 @@ __global__ __launch_bounds__(64) void k_generic(GenArgs a) {
                     const double PQa = sel3(ax, PQ[0], PQ[1], PQ[2]);
                     const double C00 = fma(ur * (-q.zeta * inv_de), PQa, sel3(ax, p.pa[0], p.pa[1], p.pa[2]));
                     const double D00 = fma(ur * (p.zeta * inv_de), PQa, sel3(ax, q.pa[0], q.pa[1], q.pa[2]));
-                    double I[7][7];
+                    double *I = I2 + lane * NI;
+                    double c2[GEN_NR], c1[GEN_NR], c0[GEN_NR];
+                    c1[0] = ax == 0 ? pref * sw[r] : 1.0;
 #pragma unroll
-                    for (int e = 0; e < 7; ++e)
-#pragma unroll
-                        for (int f = 0; f < 7; ++f) I[e][f] = 0.0;
-                    I[0][0] = ax == 0 ? pref * sw[r] : 1.0;
-#pragma unroll
-                    for (int e = 1; e < 7; ++e) {
-                        double t = C00 * I[e - 1][0];
-                        if (e >= 2) t = fma((double)(e - 1) * B10, I[e - 2][0], t);
-                        I[e][0] = e <= E ? t : 0.0;
+                    for (int e = 1; e < GEN_NR; ++e) {
+                        double t = C00 * c1[e - 1];
+                        if (e >= 2) t = fma((double)(e - 1) * B10, c1[e - 2], t);
+                        c1[e] = e <= E ? t : 0.0;
                     }
 #pragma unroll
-                    for (int f = 1; f < 7; ++f) {
-                        if (f <= F) {
-                            double t0 = D00 * I[0][f - 1];
-                            if (f >= 2) t0 = fma((double)(f - 1) * B01, I[0][f - 2], t0);
-                            I[0][f] = t0;
+                    for (int e = 0; e < GEN_NR; ++e) {
+                        if (e <= E) I[e * F1] = c1[e];
+                        c2[e] = 0.0;
+                    }
+                    for (int f = 1; f <= F; ++f) {
+                        const double bf = (double)(f - 1) * B01;
 #pragma unroll
-                            for (int e = 1; e < 7; ++e) {
-                                double t = fma(D00, I[e][f - 1], (double)e * B00 * I[e - 1][f - 1]);
-                                if (f >= 2) t = fma((double)(f - 1) * B01, I[e][f - 2], t);
-                                I[e][f] = e <= E ? t : 0.0;
-                            }
+                        for (int e = 0; e < GEN_NR; ++e) {
+                            double t = fma(D00, c1[e], bf * c2[e]);
+                            if (e >= 1) t = fma((double)e * B00, c1[e - 1], t);
+                            c0[e] = e <= E ? t : 0.0;
+                        }
+#pragma unroll
+                        for (int e = 0; e < GEN_NR; ++e) {
+                            if (e <= E) I[e * F1 + f] = c0[e];
+                            c2[e] = c1[e];
+                            c1[e] = c0[e];
                         }
                     }
+                }
+                __syncthreads();
+                // transferred factors G[a][b][c][d] = sum_k sum_l C(b,k) AB^(b-k)
+                // C(d,l) CD^(d-l) I[a+k][c+l], spread over all lanes
+                for (int t = lane; t < n * 3 * NG; t += 64) {
+                    const int ra = t / NG;
+                    int g = t - ra * NG;
+                    const int id = g % D4;
+                    g /= D4;
+                    const int ic = g % D3;
+                    g /= D3;
+                    const int ib = g % D2;
+                    const int ia = g / D2;
+                    const int ax = ra % 3;
                     const double ABq = sel3(ax, pb.ab[0], pb.ab[1], pb.ab[2]);
                     const double CDq = sel3(ax, pk.ab[0], pk.ab[1], pk.ab[2]);
-                    double *g = G + (r * 3 + ax) * GEN_G;
-#pragma unroll
-                    for (int ia = 0; ia < GEN_D; ++ia)
-#pragma unroll
-                        for (int ib = 0; ib < GEN_D; ++ib)
-#pragma unroll
-                            for (int ic = 0; ic < GEN_D; ++ic)
-#pragma unroll
-                                for (int id = 0; id < GEN_D; ++id) {
-                                    if (ia > la || ib > lb || ic > lc || id > ld) continue;
-                                    double s = 0.0, pab = 1.0;
-#pragma unroll
-                                    for (int k = ib; k >= 0; --k) {  // (A-B)^(ib-k) C(ib,k)
-                                        double scd = 0.0, pcd = 1.0;
-#pragma unroll
-                                        for (int l = id; l >= 0; --l) {
-                                            scd = fma(binom(id, l) * pcd, I[ia + k][ic + l], scd);
-                                            pcd *= CDq;
-                                        }
-                                        s = fma(binom(ib, k) * pab, scd, s);
-                                        pab *= ABq;
-                                    }
-                                    g[((ia * GEN_D + ib) * GEN_D + ic) * GEN_D + id] = s;
-                                }
+                    const double *I = I2 + ra * NI;
+                    double s = 0.0, pab = 1.0;
+                    for (int k = ib; k >= 0; --k) {
+                        double scd = 0.0, pcd = 1.0;
+                        for (int l = id; l >= 0; --l) {
+                            scd = fma(binom(id, l) * pcd, I[(ia + k) * F1 + ic + l], scd);
+                            pcd *= CDq;
+                        }
+                        s = fma(binom(ib, k) * pab, scd, s);
+                        pab *= ABq;
+                    }
+                    G[t] = s;
                 }
                 __syncthreads();
                 for (int i = lane; i < NABCD; i += 64) {
@@ -213,11 +240,11 @@ __global__ __launch_bounds__(64) void k_generic(GenArgs a) {
                     int o[3];
 #pragma unroll
                     for (int q3 = 0; q3 < 3; ++q3)
-                        o[q3] = ((comp_axis(la, ia, q3) * GEN_D + comp_axis(lb, ib, q3)) * GEN_D +
-                                 comp_axis(lc, ic, q3)) * GEN_D + comp_axis(ld, id, q3);
+                        o[q3] = ((kCartTab[la][ia][q3] * D2 + kCartTab[lb][ib][q3]) * D3 + kCartTab[lc][ic][q3]) * D4 +
+                                kCartTab[ld][id][q3];
                     double s = 0.0;
                     for (int r = 0; r < n; ++r)
-                        s = fma(G[(r * 3) * GEN_G + o[0]] * G[(r * 3 + 1) * GEN_G + o[1]], G[(r * 3 + 2) * GEN_G + o[2]], s);
+                        s = fma(G[(r * 3) * NG + o[0]] * G[(r * 3 + 1) * NG + o[1]], G[(r * 3 + 2) * NG + o[2]], s);
                     acc[i] += s;
                 }
                 __syncthreads();
@@ -247,61 +274,42 @@ __global__ __launch_bounds__(64) void k_generic(GenArgs a) {
         }
         __syncthreads();
         auto Pv = [&](int r, int c) { return P[(size_t)r * nb + c]; };
-        // A_ab += 2 sum_cd v P_cd
-        for (int ab = lane; ab < NA * NB; ab += 64) {
-            double s = 0.0;
-            for (int cd = 0; cd < NCD; ++cd) s = fma(acc[ab * NCD + cd], Pv(pk.f0 + cd / ND, pk.f1 + cd % ND), s);
-            unsafeAtomicAdd(&Am[(size_t)(pb.f0 + ab / NB) * nb + pb.f1 + ab % NB], 2.0 * s);
-        }
-        // A_cd += 2 sum_ab v P_ab
-        for (int cd = lane; cd < NCD; cd += 64) {
-            double s = 0.0;
-            for (int ab = 0; ab < NA * NB; ++ab) s = fma(acc[ab * NCD + cd], Pv(pb.f0 + ab / NB, pb.f1 + ab % NB), s);
-            unsafeAtomicAdd(&Am[(size_t)(pk.f0 + cd / ND) * nb + pk.f1 + cd % ND], 2.0 * s);
-        }
+        // A_ab += 2 sum_cd v P_cd ; A_cd += 2 sum_ab v P_ab
+        seg_reduce(lane, NA * NB, NCD, [&](int ab, int cd) {
+            return acc[ab * NCD + cd] * Pv(pk.f0 + cd / ND, pk.f1 + cd % ND);
+        }, [&](int ab, double v) { unsafeAtomicAdd(&Am[(size_t)(pb.f0 + ab / NB) * nb + pb.f1 + ab % NB], 2.0 * v); });
+        seg_reduce(lane, NCD, NA * NB, [&](int cd, int ab) {
+            return acc[ab * NCD + cd] * Pv(pb.f0 + ab / NB, pb.f1 + ab % NB);
+        }, [&](int cd, double v) { unsafeAtomicAdd(&Am[(size_t)(pk.f0 + cd / ND) * nb + pk.f1 + cd % ND], 2.0 * v); });
         // B_ac += sum_bd v P_bd ; B_bc += sum_ad v P_ad ; B_ad += sum_bc v P_bc ; B_bd += sum_ac v P_ac
-        for (int t = lane; t < NA * NC; t += 64) {
-            const int ia = t / NC, ic = t % NC;
-            double s = 0.0;
-            for (int ib = 0; ib < NB; ++ib)
-                for (int id = 0; id < ND; ++id)
-                    s = fma(acc[(ia * NB + ib) * NCD + ic * ND + id], Pv(pb.f1 + ib, pk.f1 + id), s);
-            unsafeAtomicAdd(&Bm[(size_t)(pb.f0 + ia) * nb + pk.f0 + ic], s);
-        }
-        for (int t = lane; t < NB * NC; t += 64) {
-            const int ib = t / NC, ic = t % NC;
-            double s = 0.0;
-            for (int ia = 0; ia < NA; ++ia)
-                for (int id = 0; id < ND; ++id)
-                    s = fma(acc[(ia * NB + ib) * NCD + ic * ND + id], Pv(pb.f0 + ia, pk.f1 + id), s);
-            unsafeAtomicAdd(&Bm[(size_t)(pb.f1 + ib) * nb + pk.f0 + ic], s);
-        }
-        for (int t = lane; t < NA * ND; t += 64) {
-            const int ia = t / ND, id = t % ND;
-            double s = 0.0;
-            for (int ib = 0; ib < NB; ++ib)
-                for (int ic = 0; ic < NC; ++ic)
-                    s = fma(acc[(ia * NB + ib) * NCD + ic * ND + id], Pv(pb.f1 + ib, pk.f0 + ic), s);
-            unsafeAtomicAdd(&Bm[(size_t)(pb.f0 + ia) * nb + pk.f1 + id], s);
-        }
-        for (int t = lane; t < NB * ND; t += 64) {
-            const int ib = t / ND, id = t % ND;
-            double s = 0.0;
-            for (int ia = 0; ia < NA; ++ia)
-                for (int ic = 0; ic < NC; ++ic)
-                    s = fma(acc[(ia * NB + ib) * NCD + ic * ND + id], Pv(pb.f0 + ia, pk.f0 + ic), s);
-            unsafeAtomicAdd(&Bm[(size_t)(pb.f1 + ib) * nb + pk.f1 + id], s);
-        }
+        seg_reduce(lane, NA * NC, NB * ND, [&](int o, int t) {
+            const int ia = o / NC, ic = o % NC, ib = t / ND, id = t % ND;
+            return acc[(ia * NB + ib) * NCD + ic * ND + id] * Pv(pb.f1 + ib, pk.f1 + id);
+        }, [&](int o, double v) { unsafeAtomicAdd(&Bm[(size_t)(pb.f0 + o / NC) * nb + pk.f0 + o % NC], v); });
+        seg_reduce(lane, NB * NC, NA * ND, [&](int o, int t) {
+            const int ib = o / NC, ic = o % NC, ia = t / ND, id = t % ND;
+            return acc[(ia * NB + ib) * NCD + ic * ND + id] * Pv(pb.f0 + ia, pk.f1 + id);
+        }, [&](int o, double v) { unsafeAtomicAdd(&Bm[(size_t)(pb.f1 + o / NC) * nb + pk.f0 + o % NC], v); });
+        seg_reduce(lane, NA * ND, NB * NC, [&](int o, int t) {
+            const int ia = o / ND, id = o % ND, ib = t / NC, ic = t % NC;
+            return acc[(ia * NB + ib) * NCD + ic * ND + id] * Pv(pb.f1 + ib, pk.f0 + ic);
+        }, [&](int o, double v) { unsafeAtomicAdd(&Bm[(size_t)(pb.f0 + o / ND) * nb + pk.f1 + o % ND], v); });
+        seg_reduce(lane, NB * ND, NA * NC, [&](int o, int t) {
+            const int ib = o / ND, id = o % ND, ia = t / NC, ic = t % NC;
+            return acc[(ia * NB + ib) * NCD + ic * ND + id] * Pv(pb.f0 + ia, pk.f0 + ic);
+        }, [&](int o, double v) { unsafeAtomicAdd(&Bm[(size_t)(pb.f1 + o / ND) * nb + pk.f1 + o % ND], v); });
         __syncthreads();
     }
 }
 
-// dynamic LDS of the generic kernel for a class: G for n roots + accumulators
+// dynamic LDS of the generic kernel for a class: 2-D integrals and transferred
+// factors for n roots x 3 axes, and the quartet's accumulators
 inline size_t generic_lds_bytes(int la, int lb, int lc, int ld) {
     const size_t nabcd = (size_t)((la + 1) * (la + 2) / 2) * ((lb + 1) * (lb + 2) / 2) * ((lc + 1) * (lc + 2) / 2) *
                          ((ld + 1) * (ld + 2) / 2);
-    const int n = (la + lb + lc + ld) / 2 + 1;
-    return ((size_t)n * 3 * GEN_G + nabcd) * sizeof(double);
+    const size_t n = (la + lb + lc + ld) / 2 + 1;
+    const size_t ni = (size_t)(la + lb + 1) * (lc + ld + 1), ng = (size_t)(la + 1) * (lb + 1) * (lc + 1) * (ld + 1);
+    return (n * 3 * (ni + ng) + nabcd) * sizeof(double);
 }
 
 }  // namespace pk

@@ -327,6 +327,9 @@ int build_work(pk_ctx *c) {
             const int cls = quart_class(cb, ck);
             const ClassFns &f = kClasses[cls];
             const bool split = mixed && f.has_tmpl;
+            // kets per work item: a wave of 64 lanes for the per-lane kernels,
+            // a few quartets per workgroup for the cooperative kernel
+            const size_t group = f.has_tmpl ? 64 : 4;
             for (int prec : {PK_PREC_FP64, PK_PREC_MIXED}) {
                 if (prec == PK_PREC_MIXED && !split) continue;
                 ClassWork w;
@@ -362,10 +365,10 @@ int build_work(pk_ctx *c) {
                         std::stable_sort(tmp.begin(), tmp.end(),
                                          [&](int x, int y) { return c->pairs[x].nprim > c->pairs[y].nprim; });
                         const int kb = c->pairs[b].nprim;
-                        for (size_t s = 0; s < n; s += 64) {
+                        for (size_t s = 0; s < n; s += group) {
                             wb.push_back(b);
                             ws.push_back((int)kets.size() + (int)s);
-                            wc.push_back((int)std::min<size_t>(64, n - s));
+                            wc.push_back((int)std::min<size_t>(group, n - s));
                             wm.push_back(m);
                         }
                         for (size_t s = 0; s < n; ++s) {

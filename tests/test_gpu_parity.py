@@ -81,27 +81,37 @@ def test_jk_matches_oracle(basis, mol, tol, engine):
 
 
 @pytest.mark.parametrize("cutoff", [1e-4, 1e-6])
-@pytest.mark.parametrize("basis,mol", [("cc-pvdz", "propane"), ("cc-pvtz", "water")])
+@pytest.mark.parametrize("basis,mol", [("cc-pvdz", "propane"), ("cc-pvtz", "propane")])
 def test_mixed_precision_jk(basis, mol, cutoff):
     """Mixed precision: quartets with Schwarz bound below the cutoff in FP32.
     Each FP32 integral carries a relative error of a few 1e-7 on a value below
-    the cutoff, so J and K stay within 1e-6 * cutoff * (number of terms) of
-    FP64; the bar here is 1e-5 * cutoff * nb^2 / 100, and FP64 quartets only
-    (cutoff 0) must still meet the FP64 bar."""
+    the cutoff; the bar is 1e-5 * cutoff * nb^2 / 100 on J and K.  With cutoff
+    0 (FP64 quartets only) the FP64 bar holds.  Reference: the oracle for
+    cc-pVDZ; for cc-pVTZ (225 functions, f shells) the FP64 engine, itself
+    pinned against the oracle on the f-shell cases above."""
     from pkfock.engine import PREC_MIXED
     Z, X = _molecule(mol)
-    ref, eng = _pair(basis, Z, X, tol=1e-12)
-    P = _density(ref.nbasis, 11)
-    Jr, Kr = ref.jk(P)
-    eng.set("os", 1e-12, precision=PREC_MIXED, fp32_cutoff=0.0)
+    tol = 1e-12
+    if basis == "cc-pvdz":
+        ref, eng = _pair(basis, Z, X, tol=tol)
+        nb = ref.nbasis
+        P = _density(nb, 11)
+        Jr, Kr = ref.jk(P)
+    else:
+        from pkfock.host import Molecule
+        eng = FockEngine(Molecule(Z, X, basis).shells(), "os", tol)
+        nb = eng.nbasis
+        P = _density(nb, 11)
+        Jr, Kr = eng.jk(P)
+    eng.set("os", tol, precision=PREC_MIXED, fp32_cutoff=0.0)
     J, K = eng.jk(P)
     assert max(np.abs(J - Jr).max(), np.abs(K - Kr).max()) < FOCK_TOL
-    eng.set("os", 1e-12, precision=PREC_MIXED, fp32_cutoff=cutoff)
+    eng.set("os", tol, precision=PREC_MIXED, fp32_cutoff=cutoff)
     J, K = eng.jk(P)
     dj, dk = np.abs(J - Jr).max(), np.abs(K - Kr).max()
     n32 = sum(c["quartets"] for c in eng.class_stats() if c["precision"])
     print("%s/%s cutoff %g: %d FP32 quartets of %d  max|dJ| = %.3e  max|dK| = %.3e" % (
         mol, basis, cutoff, n32, eng.stats()["quartets"], dj, dk))
     assert n32 > 0
-    bar = 1e-5 * cutoff * ref.nbasis ** 2 / 100
+    bar = 1e-5 * cutoff * nb ** 2 / 100
     assert dj < bar and dk < bar
