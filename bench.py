@@ -75,6 +75,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="c20h42-ccpvdz")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 512 MiB device to device 3 times (scales rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--fp32-cutoff", type=float, default=None,
                     help="mixed precision: quartets with Schwarz bound below this run in FP32")
     args = ap.parse_args()
@@ -84,6 +86,16 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.copy_calibration:
+        n = 64 * 1024 * 1024
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n * 8, "copies": 3}), flush=True)
+        return
     from pkfock.host import Molecule
     from pkfock.molecules import alkane
 

@@ -49,6 +49,7 @@ int fail(int code, const std::string &msg) {
     } while (0)
 
 constexpr int kMaxL = PK_MAX_L;
+constexpr double kPrimCut = 1e-17;  // primitive-pair screening bound (see add_molecule)
 constexpr int kPairClasses = (kMaxL + 1) * (kMaxL + 2) / 2;
 constexpr int kQuartClasses = kPairClasses * (kPairClasses + 1) / 2;
 
@@ -610,6 +611,9 @@ int add_molecule(pk_ctx *c, const pk_basis_desc *d, int m) {
     c->n2 += (size_t)f0 * f0;
     c->nbmax = std::max(c->nbmax, f0);
     // shell pairs a >= b, first shell carries the larger l (types.h:606-675 pair data)
+    const size_t pair0 = c->pairs.size();
+    std::vector<std::vector<pk::PrimDev>> pp_all;
+    double kmax = 0.0;  // max over primitive pairs of |kc| / sqrt(zeta)
     for (int A = s0; A < c->ns; ++A)
         for (int Bs = s0; Bs <= A; ++Bs) {
             const int sa = c->sh_l[A] >= c->sh_l[Bs] ? A : Bs, sb = sa == A ? Bs : A;
@@ -617,11 +621,10 @@ int add_molecule(pk_ctx *c, const pk_basis_desc *d, int m) {
             p.f0 = c->sh_f0[sa];
             p.f1 = c->sh_f0[sb];
             p.diag = A == Bs;
-            p.prim_off = (int)c->prims.size();
-            p.nprim = c->sh_np[sa] * c->sh_np[sb];
             const double *Ac = &c->sh_c[3 * sa], *Bc = &c->sh_c[3 * sb];
             for (int q = 0; q < 3; ++q) p.ab[q] = Ac[q] - Bc[q];
             const double R2 = p.ab[0] * p.ab[0] + p.ab[1] * p.ab[1] + p.ab[2] * p.ab[2];
+            std::vector<pk::PrimDev> pps;
             for (int i = 0; i < c->sh_np[sa]; ++i)
                 for (int j = 0; j < c->sh_np[sb]; ++j) {
                     const double al = c->ex[c->sh_poff[sa] + i], be = c->ex[c->sh_poff[sb] + j];
@@ -638,13 +641,26 @@ int add_molecule(pk_ctx *c, const pk_basis_desc *d, int m) {
                         pp.p[q] = (al * Ac[q] + be * Bc[q]) * iz;
                         pp.pa[q] = pp.p[q] - Ac[q];
                     }
-                    c->prims.push_back(pp);
+                    kmax = std::max(kmax, std::fabs(pp.kc) / std::sqrt(pp.zeta));
+                    pps.push_back(pp);
                 }
+            pp_all.push_back(std::move(pps));
             c->pairs.push_back(p);
             c->pair_cls.push_back(pk::pair_class(c->sh_l[sa], c->sh_l[sb]));
             c->pair_q.push_back(0.0);
             c->pair_mol.push_back(m);
         }
+    // Primitive screening: a primitive pair whose prefactor bound
+    // |kc_p| max_q |kc_q| / sqrt(eta_q) (>= |kc_p kc_q| / sqrt(zeta_p + eta_q))
+    // is below kPrimCut cannot move any integral by more than that times its
+    // angular factors; it is dropped (distant tight primitives: e^{-ab R^2/z}).
+    for (size_t k = 0; k < pp_all.size(); ++k) {
+        pk::PairDev &p = c->pairs[pair0 + k];
+        p.prim_off = (int)c->prims.size();
+        for (const pk::PrimDev &pp : pp_all[k])
+            if (std::fabs(pp.kc) * kmax >= kPrimCut) c->prims.push_back(pp);
+        p.nprim = (int)c->prims.size() - p.prim_off;
+    }
     return PK_OK;
 }
 
