@@ -147,6 +147,28 @@ __global__ __launch_bounds__(256) void k_jk(JKArgs a) {
     sfor<0, NA>([&](auto i_) { kac[PK_CI(i_)] = 0.0; sfor<0, ND>([&](auto d_) { kad[PK_CI(i_)][PK_CI(d_)] = 0.0; }); });
     sfor<0, NB>([&](auto i_) { kbc[PK_CI(i_)] = 0.0; sfor<0, ND>([&](auto d_) { kbd[PK_CI(i_)][PK_CI(d_)] = 0.0; }); });
 
+    // The work lists order each bra's kets by primitive count, then first
+    // shell, so lanes with equal (primitive count, first ket shell) are
+    // contiguous; (a|c) and (b|c) exchange sums are added over such a segment
+    // (segmented scan over the wave) and stored once, by its last lane.
+    const int ckey = active ? (pk.nprim << 20) + pk.f0 : -1 - lane;
+    bool seg_same[6];
+    sfor<0, 6>([&](auto k_) {
+        constexpr int off = 1 << PK_CI(k_);
+        // the shuffle runs on every lane (a source lane outside the exec mask reads as 0)
+        const int up = __shfl_up(ckey, off, 64);
+        seg_same[PK_CI(k_)] = lane >= off && up == ckey;
+    });
+    const int down = __shfl_down(ckey, 1, 64);
+    const bool seg_tail = lane == 63 || down != ckey;
+    auto seg_sum = [&](double v) {
+        sfor<0, 6>([&](auto k_) {
+            const double o = __shfl_up(v, 1 << PK_CI(k_), 64);
+            if (seg_same[PK_CI(k_)]) v += o;
+        });
+        return v;
+    };
+
     // digestion of one ket component (ic, id) with its bra block v[NAB]
     auto digest = [&](auto jj_, const R (&v)[NAB]) {
         constexpr int jj = PK_CI(jj_);
@@ -174,16 +196,17 @@ __global__ __launch_bounds__(256) void k_jk(JKArgs a) {
             });
         });
         if (active) unsafeAtomicAdd(&Am[(size_t)cf * nb + df], 2.0 * jcd);
-        // last ket component of this c in the chunk: flush the (a|c), (b|c) sums
+        // last ket component of this c in the chunk: flush the (a|c), (b|c)
+        // sums, added first over the lanes that share the ket's first shell
         if constexpr (id == ND - 1 || jj == NCH - 1) {
-            if (active) {
-                sfor<0, NA>([&](auto ia_) {
-                    unsafeAtomicAdd(&Bm[(size_t)(pb.f0 + PK_CI(ia_)) * nb + cf], kac[PK_CI(ia_)]);
-                });
-                sfor<0, NB>([&](auto ib_) {
-                    unsafeAtomicAdd(&Bm[(size_t)(pb.f1 + PK_CI(ib_)) * nb + cf], kbc[PK_CI(ib_)]);
-                });
-            }
+            sfor<0, NA>([&](auto ia_) {
+                const double v = seg_sum(kac[PK_CI(ia_)]);
+                if (active && seg_tail) unsafeAtomicAdd(&Bm[(size_t)(pb.f0 + PK_CI(ia_)) * nb + cf], v);
+            });
+            sfor<0, NB>([&](auto ib_) {
+                const double v = seg_sum(kbc[PK_CI(ib_)]);
+                if (active && seg_tail) unsafeAtomicAdd(&Bm[(size_t)(pb.f1 + PK_CI(ib_)) * nb + cf], v);
+            });
             sfor<0, NA>([&](auto i_) { kac[PK_CI(i_)] = 0.0; });
             sfor<0, NB>([&](auto i_) { kbc[PK_CI(i_)] = 0.0; });
         }

@@ -306,7 +306,7 @@ int build_schwarz(pk_ctx *c) {
 // Screened work lists: for every quartet class (cb >= ck), every molecule and
 // every bra pair, the ket pairs of the same molecule with
 // qmax_bra * qmax_ket >= tol (and ket <= bra when cb == ck), sorted by
-// primitive count, cut into waves of at most 64 kets; the waves of all
+// primitive count and shells, cut into waves of at most 64 kets; the waves of all
 // molecules form one list per class.  In mixed precision the kets with
 // qmax_bra * qmax_ket < cutoff form a second, FP32 list of the class (classes
 // of the cooperative kernel stay FP64).
@@ -363,8 +363,15 @@ int build_work(pk_ctx *c) {
                         const size_t n = k1 - k0;
                         if (n == 0) continue;
                         tmp.assign(Kl.begin() + k0, Kl.begin() + k1);
-                        std::stable_sort(tmp.begin(), tmp.end(),
-                                         [&](int x, int y) { return c->pairs[x].nprim > c->pairs[y].nprim; });
+                        // primitive count descending (lanes of a wave finish together),
+                        // then first and second shell: kets sharing a first shell sit in
+                        // neighbouring lanes, whose exchange sums the kernels combine
+                        std::sort(tmp.begin(), tmp.end(), [&](int x, int y) {
+                            const pk::PairDev &px = c->pairs[x], &py = c->pairs[y];
+                            if (px.nprim != py.nprim) return px.nprim > py.nprim;
+                            if (px.f0 != py.f0) return px.f0 < py.f0;
+                            return px.f1 < py.f1;
+                        });
                         const int kb = c->pairs[b].nprim;
                         for (size_t s = 0; s < n; s += group) {
                             wb.push_back(b);
@@ -603,6 +610,10 @@ int add_molecule(pk_ctx *c, const pk_basis_desc *d, int m) {
         off += np;
         f0 += pk::ncart(l);
     }
+    // the kernels pack (pair primitive count, first function) into one int
+    if (f0 >= (1 << 20)) return fail(PK_ERR_ARG, "pk_ctx_create: more than 2^20 functions in one molecule");
+    for (int s = s0; s < s0 + d->nshells; ++s)
+        if (c->sh_np[s] * c->sh_np[s] >= 2048) return fail(PK_ERR_ARG, "pk_ctx_create: shell with more than 45 primitives");
     c->ns += d->nshells;
     c->mol_foff.push_back(c->nb);
     c->mol_nb.push_back(f0);
