@@ -13,11 +13,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "eri_classes.h"
@@ -303,6 +305,25 @@ int build_schwarz(pk_ctx *c) {
     return PK_OK;
 }
 
+// Runs f(i) for i in [0, n) on up to 16 host threads (dynamic chunks of 64).
+template <typename F>
+void parallel_for(size_t n, F f) {
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const unsigned nt = (unsigned)std::min<size_t>(hw, (n + 63) / 64);
+    if (nt <= 1) {
+        for (size_t i = 0; i < n; ++i) f(i);
+        return;
+    }
+    std::atomic<size_t> next{0};
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nt; ++t)
+        th.emplace_back([&] {
+            for (size_t i0; (i0 = next.fetch_add(64)) < n;)
+                for (size_t i = i0; i < std::min(n, i0 + 64); ++i) f(i);
+        });
+    for (std::thread &t : th) t.join();
+}
+
 // Screened work lists: for every quartet class (cb >= ck), every molecule and
 // every bra pair, the ket pairs of the same molecule with
 // qmax_bra * qmax_ket >= tol (and ket <= bra when cb == ck), sorted by
@@ -319,10 +340,20 @@ int build_work(pk_ctx *c) {
     for (auto &mv : by_cls)
         for (auto &v : mv)
             std::stable_sort(v.begin(), v.end(), [&](int x, int y) { return c->pair_q[x] > c->pair_q[y]; });
+    // ket order inside a wave: primitive count descending, then first and
+    // second function (one 64-bit key per pair)
+    std::vector<uint64_t> okey(np);
+    for (int p = 0; p < np; ++p)
+        okey[p] = ((uint64_t)(2047 - c->pairs[p].nprim) << 40) | ((uint64_t)c->pairs[p].f0 << 20) | (uint64_t)c->pairs[p].f1;
     c->st.quartets = 0;
     c->st.primitive_quartets = 0;
     c->st.model_flops = 0.0;
     const bool mixed = c->precision == PK_PREC_MIXED;
+    struct BraOut {
+        std::vector<int> kets;
+        int64_t nprimq = 0;
+        double flops = 0.0;
+    };
     for (int cb = 0; cb < kPairClasses; ++cb)
         for (int ck = 0; ck <= cb; ++ck) {
             const int cls = quart_class(cb, ck);
@@ -331,64 +362,72 @@ int build_work(pk_ctx *c) {
             // kets per work item: a wave of 64 lanes for the per-lane kernels,
             // a few quartets per workgroup for the cooperative kernel
             const size_t group = f.has_tmpl ? 64 : 4;
+            // bra tasks of every molecule, in order
+            std::vector<std::pair<int, int>> tasks;
+            for (int m = 0; m < c->nmol; ++m)
+                if (!by_cls[m][cb].empty() && !by_cls[m][ck].empty())
+                    for (int ib = 0; ib < (int)by_cls[m][cb].size(); ++ib) tasks.emplace_back(m, ib);
+            if (tasks.empty()) continue;
             for (int prec : {PK_PREC_FP64, PK_PREC_MIXED}) {
                 if (prec == PK_PREC_MIXED && !split) continue;
+                std::vector<BraOut> outs(tasks.size());
+                parallel_for(tasks.size(), [&](size_t t) {
+                    const int m = tasks[t].first, ib = tasks[t].second;
+                    const std::vector<int> &B = by_cls[m][cb], &Kl = by_cls[m][ck];
+                    const int b = B[ib];
+                    const double qb = c->pair_q[b];
+                    // number of kets (q descending) with qb * qk >= thr
+                    auto count = [&](double thr) {
+                        size_t lo = 0, hi = Kl.size();
+                        while (lo < hi) {
+                            const size_t mid = (lo + hi) / 2;
+                            if (qb * c->pair_q[Kl[mid]] >= thr)
+                                lo = mid + 1;
+                            else
+                                hi = mid;
+                        }
+                        return cb == ck ? std::min(lo, (size_t)ib + 1) : lo;
+                    };
+                    const size_t n_tol = count(c->tol);
+                    const size_t n_hi = split ? std::min(count(c->cutoff), n_tol) : n_tol;
+                    const size_t k0 = prec == PK_PREC_FP64 ? 0 : n_hi;
+                    const size_t k1 = prec == PK_PREC_FP64 ? n_hi : n_tol;
+                    if (k1 <= k0) return;
+                    std::vector<std::pair<uint64_t, int>> kk(k1 - k0);
+                    for (size_t i = k0; i < k1; ++i) kk[i - k0] = {okey[Kl[i]], Kl[i]};
+                    std::sort(kk.begin(), kk.end());
+                    BraOut &o = outs[t];
+                    o.kets.resize(kk.size());
+                    const int kb = c->pairs[b].nprim;
+                    for (size_t i = 0; i < kk.size(); ++i) {
+                        o.kets[i] = kk[i].second;
+                        const double K = (double)kb * c->pairs[kk[i].second].nprim;
+                        o.nprimq += (int64_t)K;
+                        o.flops += pk::model_flops(f.la, f.lb, f.lc, f.ld, K);
+                    }
+                });
                 ClassWork w;
                 w.cls = cls;
                 w.prec = prec;
                 std::vector<int> wb, ws, wc, wm, kets;
-                std::vector<int> tmp;
-                for (int m = 0; m < c->nmol; ++m) {
-                    const std::vector<int> &B = by_cls[m][cb], &Kl = by_cls[m][ck];
-                    if (B.empty() || Kl.empty()) continue;
-                    for (size_t ib = 0; ib < B.size(); ++ib) {
-                        const int b = B[ib];
-                        const double qb = c->pair_q[b];
-                        // number of kets (q descending) with qb * qk >= thr
-                        auto count = [&](double thr) {
-                            size_t lo = 0, hi = Kl.size();
-                            while (lo < hi) {
-                                const size_t mid = (lo + hi) / 2;
-                                if (qb * c->pair_q[Kl[mid]] >= thr)
-                                    lo = mid + 1;
-                                else
-                                    hi = mid;
-                            }
-                            return cb == ck ? std::min(lo, ib + 1) : lo;
-                        };
-                        const size_t n_tol = count(c->tol);
-                        const size_t n_hi = split ? std::min(count(c->cutoff), n_tol) : n_tol;
-                        const size_t k0 = prec == PK_PREC_FP64 ? 0 : n_hi;
-                        const size_t k1 = prec == PK_PREC_FP64 ? n_hi : n_tol;
-                        const size_t n = k1 - k0;
-                        if (n == 0) continue;
-                        tmp.assign(Kl.begin() + k0, Kl.begin() + k1);
-                        // primitive count descending (lanes of a wave finish together),
-                        // then first and second shell: kets sharing a first shell sit in
-                        // neighbouring lanes, whose exchange sums the kernels combine
-                        std::sort(tmp.begin(), tmp.end(), [&](int x, int y) {
-                            const pk::PairDev &px = c->pairs[x], &py = c->pairs[y];
-                            if (px.nprim != py.nprim) return px.nprim > py.nprim;
-                            if (px.f0 != py.f0) return px.f0 < py.f0;
-                            return px.f1 < py.f1;
-                        });
-                        const int kb = c->pairs[b].nprim;
-                        for (size_t s = 0; s < n; s += group) {
-                            wb.push_back(b);
-                            ws.push_back((int)kets.size() + (int)s);
-                            wc.push_back((int)std::min<size_t>(group, n - s));
-                            wm.push_back(m);
-                        }
-                        for (size_t s = 0; s < n; ++s) {
-                            const double K = (double)kb * c->pairs[tmp[s]].nprim;
-                            w.nprimq += (int64_t)K;
-                            w.flops += pk::model_flops(f.la, f.lb, f.lc, f.ld, K);
-                        }
-                        kets.insert(kets.end(), tmp.begin(), tmp.end());
-                        w.nquart += (int64_t)n;
+                for (size_t t = 0; t < tasks.size(); ++t) {
+                    const BraOut &o = outs[t];
+                    const size_t n = o.kets.size();
+                    if (n == 0) continue;
+                    const int b = by_cls[tasks[t].first][cb][tasks[t].second];
+                    for (size_t s = 0; s < n; s += group) {
+                        wb.push_back(b);
+                        ws.push_back((int)kets.size() + (int)s);
+                        wc.push_back((int)std::min<size_t>(group, n - s));
+                        wm.push_back(tasks[t].first);
                     }
+                    kets.insert(kets.end(), o.kets.begin(), o.kets.end());
+                    w.nquart += (int64_t)n;
+                    w.nprimq += o.nprimq;
+                    w.flops += o.flops;
                 }
                 if (w.nquart == 0) continue;
+                if (kets.size() > (size_t)INT32_MAX) return fail(PK_ERR_ARG, "build_work: more than 2^31 quartets in one class");
                 w.nwaves = (int)wb.size();
                 std::vector<int> packed;
                 packed.reserve(4 * wb.size());

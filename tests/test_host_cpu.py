@@ -13,6 +13,7 @@ import pytest
 
 import refcpu
 from pkfock import engine as eng_mod
+from pkfock import mlp as mlp_mod
 from pkfock import host
 from pkfock.molecules import ANGSTROM_TO_BOHR, alkane, water, water_cluster
 
@@ -61,13 +62,25 @@ def _declared(header):
     return sorted(set(re.findall(r"\b((?:pk|pkref)_[a-z0-9_]+)\s*\(", src)))
 
 
-@pytest.mark.parametrize("header,lib", [("pk_fock.h", eng_mod.LIB_PATH), ("pk_scf.h", host.LIB_PATH)])
+@pytest.mark.parametrize("header,lib", [("pk_fock.h", eng_mod.LIB_PATH), ("pk_scf.h", host.LIB_PATH),
+                                        ("pk_mlp.h", mlp_mod.LIB_PATH)])
 def test_c_abi_exports_every_declared_symbol(header, lib):
     so = ctypes.CDLL(lib)
     names = _declared(header)
     assert len(names) >= 8
     missing = [n for n in names if not hasattr(so, n)]
     assert not missing, missing
+
+
+def test_network_refuses_without_device():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("a HIP device is present")
+    except ImportError:
+        pass
+    with pytest.raises(mlp_mod.PkMlpError, match="no HIP device"):
+        mlp_mod.Net(4, 2, 8)
 
 
 def test_engine_refuses_without_device():
