@@ -52,6 +52,8 @@ typedef struct pk_scf_options {
     const double *guess_density;
     int32_t precision;   /* PK_PREC_FP64 (default) or PK_PREC_MIXED */
     double fp32_cutoff;  /* mixed: quartets with Schwarz bound below it run in FP32 */
+    int32_t guess_fallback; /* with use_guess: start from the core guess instead when
+                               its first DIIS error is smaller (one extra Fock build) */
 } pk_scf_options;
 
 typedef struct pk_scf_result {
@@ -62,6 +64,9 @@ typedef struct pk_scf_result {
     double first_diis_error; /* RMS of the first DIIS error matrix */
     double wall_s;           /* guess to convergence */
     double fock_ms;          /* summed device time of the Fock builds */
+    int32_t guess_used;      /* 1: the supplied guess density, 0: the core guess */
+    double guess_diis_error; /* first DIIS errors of the supplied and the core */
+    double core_diis_error;  /* guess (guess_fallback only; else 0) */
 } pk_scf_result;
 
 void pk_scf_default_options(pk_scf_options *o);

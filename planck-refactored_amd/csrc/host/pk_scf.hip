@@ -189,6 +189,7 @@ void pk_scf_default_options(pk_scf_options *o) {
     o->guess_density = nullptr;
     o->precision = PK_PREC_FP64;
     o->fp32_cutoff = 0.0;
+    o->guess_fallback = 0;
 }
 
 int pk_scf_rhf(const pk_basis *bas, const pk_scf_options *opt, pk_scf_result *res, double *P_out, double *hist,
@@ -220,6 +221,7 @@ int pk_scf_rhf(const pk_basis *bas, const pk_scf_options *opt, pk_scf_result *re
     std::vector<std::vector<double>> Bm;  // B over slots
     const double one = 1.0, zero = 0.0, two = 2.0;
     double Eprev = 0.0, Etot = 0.0;
+    bool have_G = false;  // G of the current P already built (guess comparison)
     const int g2 = (int)((n2 + 255) / 256);
     const int nparts = 256;
     std::vector<double> hpart(2 * nparts);
@@ -289,17 +291,55 @@ int pk_scf_rhf(const pk_basis *bas, const pk_scf_options *opt, pk_scf_result *re
 
     if (opt->use_guess && opt->guess_density) {
         SCF_HIP(hipMemcpyAsync(P, opt->guess_density, n2 * 8, hipMemcpyHostToDevice, st));
+        res->guess_used = 1;
     } else if ((rc = density(H, P)) != PK_OK) {
         goto done;
     }
+    if (opt->use_guess && opt->guess_density && opt->guess_fallback) {
+        // first DIIS error of each candidate: RMS of X (F P S - S P F) X with
+        // F = H + G[P]; the core guess replaces the supplied one when smaller
+        auto first_error = [&](const double *Pc, double *Gc, double &err) -> int {
+            if (pk_fock_2e(ctx, Pc, Gc, PK_MEM_DEVICE) != PK_OK) return pkh::fail(PK_ERR_HIP, pk_last_error());
+            pk_stats s;
+            pk_ctx_stats(ctx, &s);
+            res->fock_ms += s.last_build_ms;
+            hipLaunchKernelGGL(k_add, dim3(g2), dim3(256), 0, st, H, Gc, F, n2);
+            if (gemm(rocblas_operation_none, rocblas_operation_none, n, 1.0, F, Pc, W1) != rocblas_status_success ||
+                gemm(rocblas_operation_none, rocblas_operation_none, n, 1.0, W1, S, M) != rocblas_status_success)
+                return pkh::fail(PK_ERR_HIP, "gemm failed");
+            hipLaunchKernelGGL(k_antisym, dim3(g2), dim3(256), 0, st, M, D, n);
+            if (gemm(rocblas_operation_none, rocblas_operation_none, n, 1.0, X, D, W1) != rocblas_status_success ||
+                gemm(rocblas_operation_none, rocblas_operation_none, n, 1.0, W1, X, M) != rocblas_status_success)
+                return pkh::fail(PK_ERR_HIP, "gemm failed");
+            double v = 0;
+            if (rocblas_set_pointer_mode(hb, rocblas_pointer_mode_host) != rocblas_status_success ||
+                rocblas_ddot(hb, (rocblas_int)n2, M, 1, M, 1, &v) != rocblas_status_success)
+                return pkh::fail(PK_ERR_HIP, "ddot failed");
+            err = std::sqrt(v / (double)n2);
+            return PK_OK;
+        };
+        double eg = 0, ec = 0;
+        if ((rc = first_error(P, G, eg)) != PK_OK) goto done;
+        if ((rc = density(H, Pn)) != PK_OK) goto done;
+        if ((rc = first_error(Pn, C, ec)) != PK_OK) goto done;  // C is free until the first density()
+        res->guess_diis_error = eg;
+        res->core_diis_error = ec;
+        if (ec < eg) {
+            std::swap(P, Pn);
+            SCF_HIP(hipMemcpyAsync(G, C, n2 * 8, hipMemcpyDeviceToDevice, st));
+            res->guess_used = 0;
+        }
+        have_G = true;
+    }
 
     for (int it = 1; it <= max_cycles; ++it) {
-        SCF_PK(pk_fock_2e(ctx, P, G, PK_MEM_DEVICE));
-        {
+        if (!have_G) {
+            SCF_PK(pk_fock_2e(ctx, P, G, PK_MEM_DEVICE));
             pk_stats s;
             pk_ctx_stats(ctx, &s);
             res->fock_ms += s.last_build_ms;
         }
+        have_G = false;
         hipLaunchKernelGGL(k_add, dim3(g2), dim3(256), 0, st, H, G, F, n2);
         double eph = 0, epf = 0;
         SCF_RB(rocblas_set_pointer_mode(hb, rocblas_pointer_mode_host));

@@ -19,8 +19,9 @@ reference, src/basis/basis.cpp:20-37): O s s p s p d (15), H s s, H s s.
 The two networks (pkfock.mlp.Net, hand-written HIP kernels) are trained on
 converged RHF densities produced by this engine's own SCF.  At SCF time the
 learned density is used only when its first DIIS error is not larger than the
-core guess's; otherwise the core guess is used (the reference's plain guess,
-src/scf/scf.cpp:33-45).  No counterpart in the reference.
+core guess's (pk_scf_options.guess_fallback); otherwise the core guess is used
+(the reference's plain guess, src/scf/scf.cpp:33-45).  No counterpart in the
+reference.
 """
 import numpy as np
 
@@ -180,6 +181,10 @@ class LearnedGuess:
     """Two device networks (intra, inter) with output standardisation."""
 
     def __init__(self, hidden=128, seed=0, max_batch=65536):
+        import torch
+        # torch's HIP runtime is initialised before the engine's libraries run
+        # more work in this process (device buffers come from torch)
+        torch.cuda.init()
         from .mlp import Net
         self.intra = Net(N_INTRA_IN, N_INTRA_OUT, hidden, max_batch, seed)
         self.inter = Net(N_INTER_IN, N_INTER_OUT, hidden, max_batch, seed + 1)
@@ -241,6 +246,16 @@ class LearnedGuess:
         return {"intra": self.intra.params(), "inter": self.inter.params(),
                 **{"stat_%s_%d" % (k, i): v[i] for k, v in self.stats.items() for i in range(2)}}
 
+    @classmethod
+    def pretrained(cls):
+        """The network trained by pkfock.train_guess (data/water_631gs_guess.npz)."""
+        import os
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "water_631gs_guess.npz")
+        st = dict(np.load(path))
+        m = cls(hidden=128)
+        m.load_state(st)
+        return m
+
     def load_state(self, st):
         self.intra.set_params(st["intra"])
         self.inter.set_params(st["inter"])
@@ -248,14 +263,11 @@ class LearnedGuess:
 
 
 def scf_with_guess(mol, model, **scf):
-    """RHF from the learned density with the plain-guess fallback: the learned
-    density is kept when its first DIIS error is not larger than the core
-    guess's (each measured by one SCF iteration).  Returns (result, used)."""
+    """RHF from the learned density with the plain-guess fallback: the SCF
+    driver builds the Fock matrix of the learned density and of the core guess,
+    and starts from the one with the smaller first DIIS error (one extra Fock
+    build).  Returns (result, "learned" | "core")."""
     X = mol.xyz / ANGSTROM_TO_BOHR
     Pg = model.predict(X)
-    probe = dict(scf, max_cycles=1)
-    e_l = mol.rhf(guess=Pg, **probe)["first_diis_error"]
-    e_c = mol.rhf(**probe)["first_diis_error"]
-    if e_l <= e_c:
-        return mol.rhf(guess=Pg, **scf), "learned"
-    return mol.rhf(**scf), "core"
+    r = mol.rhf(guess=Pg, guess_fallback=True, **scf)
+    return r, ("learned" if r["guess_used"] else "core")

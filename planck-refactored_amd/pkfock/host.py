@@ -23,13 +23,14 @@ class ScfOptions(ctypes.Structure):
                 ("diis_dim", ctypes.c_int32), ("max_cycles", ctypes.c_int32), ("engine", ctypes.c_int32),
                 ("charge", ctypes.c_int32), ("use_guess", ctypes.c_int32),
                 ("guess_density", ctypes.POINTER(ctypes.c_double)), ("precision", ctypes.c_int32),
-                ("fp32_cutoff", ctypes.c_double)]
+                ("fp32_cutoff", ctypes.c_double), ("guess_fallback", ctypes.c_int32)]
 
 
 class ScfResult(ctypes.Structure):
     _fields_ = [("energy", ctypes.c_double), ("nuclear_repulsion", ctypes.c_double), ("iterations", ctypes.c_int32),
                 ("converged", ctypes.c_int32), ("first_diis_error", ctypes.c_double), ("wall_s", ctypes.c_double),
-                ("fock_ms", ctypes.c_double)]
+                ("fock_ms", ctypes.c_double), ("guess_used", ctypes.c_int32), ("guess_diis_error", ctypes.c_double),
+                ("core_diis_error", ctypes.c_double)]
 
 
 _lib = None
@@ -111,7 +112,7 @@ class Molecule:
         return _eng.FockEngine(self.shells(), engine=engine, tol_eri=tol_eri)
 
     def rhf(self, engine="os", tol_eri=1e-10, tol_energy=1e-10, tol_density=1e-10, diis_dim=8, max_cycles=0,
-            guess=None, fp32_cutoff=None):
+            guess=None, fp32_cutoff=None, guess_fallback=False):
         """Device RHF (pk_scf_rhf).  fp32_cutoff: mixed precision, quartets
         with Schwarz bound below it in FP32 (None: FP64 throughout)."""
         lib = load()
@@ -128,6 +129,7 @@ class Molecule:
             g = np.ascontiguousarray(guess, dtype=np.float64)
             o.use_guess = 1
             o.guess_density = g.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+            o.guess_fallback = int(bool(guess_fallback))
         r = ScfResult()
         n = self.nbasis
         P = np.zeros((n, n))

@@ -103,3 +103,22 @@ def test_learned_guess_end_to_end_small():
     print("final losses", hist[59][2], hist[-1][2], "guess used:", used, "iterations", ref["iterations"], "->",
           res["iterations"], "dE %.2e" % (res["energy"] - ref["energy"]))
     assert res["rc"] == 0 and abs(res["energy"] - ref["energy"]) < 1e-8
+
+
+def test_pretrained_guess_gain():
+    """The shipped network (trained on 2-16-water clusters) against the core
+    guess on a held-out 16-water cluster: fewer SCF iterations, same energy."""
+    from pkfock.guess import LearnedGuess, scf_with_guess
+    from pkfock.host import Molecule
+    from pkfock.molecules import water_cluster
+    model = LearnedGuess.pretrained()
+    Z, X = water_cluster(910000, n=16)
+    mol = Molecule(Z, X, "6-31g*")
+    scf = dict(tol_eri=1e-12, tol_energy=1e-8, tol_density=1e-6)
+    core = mol.rhf(**scf)
+    lrn, used = scf_with_guess(mol, model, **scf)
+    print("16 waters: core %d iterations, learned (%s) %d, first DIIS errors %.3e vs %.3e, dE %.1e" % (
+        core["iterations"], used, lrn["iterations"], lrn["guess_diis_error"], lrn["core_diis_error"],
+        lrn["energy"] - core["energy"]))
+    assert used == "learned" and lrn["iterations"] <= 0.7 * core["iterations"]
+    assert abs(lrn["energy"] - core["energy"]) < 1e-8
