@@ -16,6 +16,7 @@ What pins each part (none of these re-types the oracle's own formula):
 import json
 import math
 import os
+import sys
 import tempfile
 
 import mpmath
@@ -236,3 +237,106 @@ def test_oracle_rhf_reference_energies(case):
     assert abs(r["energy"] - case["energy"]) < 1e-9, (r["energy"], case["energy"])
     if "iterations" in case:
         assert r["iterations"] == case["iterations"]
+
+
+# ---------------------------------------------------------------- SAD guess
+sys.path.insert(0, os.path.join(os.path.dirname(HERE), "oracle"))
+import sad_ref  # noqa: E402
+
+
+def _shell_slices(ref):
+    cen, ls, _, _, _, _ = ref.shells()
+    out, f = [], 0
+    for l in ls:
+        n = (l + 1) * (l + 2) // 2
+        out.append(slice(f, f + n))
+        f += n
+    return cen, out
+
+
+def sad_parts(basis, Z, X_bohr, populations="mulliken", engine="os"):
+    ref = refcpu.Ref(basis, Z, X_bohr, engine=engine, tol_eri=1e-12)
+    cen, slices = _shell_slices(ref)
+    ao_atom = np.zeros(ref.nbasis, int)
+    for c, sl in zip(cen, slices):
+        ao_atom[sl] = int(np.argmin(np.linalg.norm(X_bohr - c, axis=1)))
+
+    def make_atom(z):
+        r = refcpu.Ref(basis, [z], np.zeros((1, 3)), engine=engine, tol_eri=1e-12)
+        return r, _shell_slices(r)[1]
+
+    return ref, sad_ref.sad_raw(list(Z), ao_atom, make_atom, populations)
+
+
+def _first_iteration(ref, P, nocc):
+    """Energy, rms / max change of the next density and DIIS error of the first
+    RHF iteration from P (scf.cpp:281-360)."""
+    S = ref.matrix("overlap")
+    H = ref.matrix("kinetic") + ref.matrix("nuclear")
+    J, K = ref.jk(P)
+    F = H + J - 0.5 * K
+    E1 = 0.5 * (P * (H + F)).sum() + ref.enuc()
+    w, U = np.linalg.eigh(S)
+    Xo = (U / np.sqrt(w)) @ U.T
+    e = Xo @ (F @ P @ S - S @ P @ F) @ Xo
+    _, V = np.linalg.eigh(Xo @ F @ Xo)
+    C = Xo @ V[:, :nocc]
+    d = 2 * C @ C.T - P
+    return E1, np.sqrt((d * d).mean()), np.abs(d).max(), np.sqrt((e * e).mean())
+
+
+def test_sad_oracle_reproduces_reference_first_iteration():
+    """Ethylene / STO-3G, SAD guess, no symmetry: the reference's log prints the
+    first SCF iteration from the SAD density: E -65.3840017, rms(D) 1.098,
+    max(D) 4.285, DIIS 5.955e-01.  The natural occupations of the raw SAD
+    density are doubly degenerate exactly at n_occ, so the projected density
+    is fixed only up to a rotation inside that pair (the reference's result is
+    whatever its eigensolver returned); and the log was written with the
+    diagonal-block shell populations.  The pin: some rotation angle of the
+    pair reproduces all four printed numbers at once."""
+    Z = [6, 6, 1, 1, 1, 1]
+    X = np.array([[0.0, 0.0, 0.0], [0.0, 0.0, 1.34], [0.92, 0.0, -0.54], [-0.92, 0.0, -0.54],
+                  [0.0, 0.92, 1.88], [0.0, -0.92, 1.88]]) * ANGSTROM_TO_BOHR
+    ref, Praw = sad_parts("sto-3g", Z, X, populations="block")
+    S = ref.matrix("overlap")
+    Pp, w, U, Xs = sad_ref.project(Praw, S, 16)
+    assert abs(np.trace(Pp @ S) - 16.0) < 1e-10
+    assert abs(w[6] - w[5]) < 1e-10 and w[7] - w[6] > 0.1  # degenerate pair at the boundary
+
+    def at(t):
+        v = np.cos(t) * U[:, 5] + np.sin(t) * U[:, 6]
+        C = Xs @ np.column_stack([v] + [U[:, k] for k in range(7, 14)])
+        return _first_iteration(ref, 2 * C @ C.T, 8)
+
+    ts = np.linspace(0.0, np.pi, 721)
+    E = np.array([at(t)[0] for t in ts])
+    i = int(np.argmin(np.abs(E + 65.3840017)))
+    lo, hi = ts[max(i - 1, 0)], ts[min(i + 1, len(ts) - 1)]
+    for _ in range(60):  # bisection on E(t) + 65.3840017 around the crossing
+        mid = 0.5 * (lo + hi)
+        if (at(lo)[0] + 65.3840017) * (at(mid)[0] + 65.3840017) <= 0:
+            hi = mid
+        else:
+            lo = mid
+    E1, rms, dmax, err = at(0.5 * (lo + hi))
+    print("theta %.4f: E1 %.8f rms(D) %.4f max(D) %.4f DIIS %.5f" % (0.5 * (lo + hi), E1, rms, dmax, err))
+    assert abs(E1 + 65.3840017) < 1e-7
+    assert abs(rms - 1.098) < 5e-4 and abs(dmax - 4.285) < 5e-4 and abs(err - 0.5955) < 5e-5
+
+
+def test_sad_oracle_invariants():
+    """Raw SAD: trace = electron count, block-diagonal per atom and shell;
+    projected: idempotent in the S metric with trace N (sad.cpp:476-505); H atom
+    UHF in STO-3G gives the textbook -0.46658185 Eh."""
+    Z, X = water()
+    Xb = X * ANGSTROM_TO_BOHR
+    ref, Praw = sad_parts("6-31g*", Z, Xb)
+    S = ref.matrix("overlap")
+    assert abs(np.trace(Praw @ S) - 10.0) < 1e-9
+    P = sad_ref.project(Praw, S, 10)[0]
+    assert abs(np.trace(P @ S) - 10.0) < 1e-9
+    assert np.abs(P @ S @ P - 2.0 * P).max() < 1e-9
+    h = refcpu.Ref("sto-3g", [1], np.zeros((1, 3)), engine="os", tol_eri=1e-12)
+    Pa, Pb, Sh = sad_ref.atomic_uhf(h, 1)
+    Hh = h.matrix("kinetic") + h.matrix("nuclear")
+    assert abs((Pa * Hh).sum() - (-0.46658185)) < 1e-8
