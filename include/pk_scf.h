@@ -54,6 +54,9 @@ typedef struct pk_scf_options {
     double fp32_cutoff;  /* mixed: quartets with Schwarz bound below it run in FP32 */
     int32_t guess_fallback; /* with use_guess: start from the core guess instead when
                                its first DIIS error is smaller (one extra Fock build) */
+    int32_t sad_guess;   /* 1: SAD initial density (sad.cpp:589-643) instead of the
+                            core Hamiltonian (ignored with use_guess) */
+    int32_t guess_only;  /* 1: return the initial density in P_out, no iterations */
 } pk_scf_options;
 
 typedef struct pk_scf_result {

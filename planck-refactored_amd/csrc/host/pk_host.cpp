@@ -12,6 +12,7 @@
 // The one-electron integrals are built per shell pair with all Cartesian
 // components at once: 1-D overlap recurrences for S and T, and a Rys rule
 // (the engine's own table) for the nuclear attraction.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -346,6 +347,85 @@ int one_electron(const Basis &b, double *S, double *T, double *V) {
 }  // namespace pkh
 
 // ====================================================================== C ABI
+namespace pkh {
+
+Basis atom_basis(const Basis &b, int a) {
+    Basis o;
+    o.Z = {b.Z[a]};
+    o.xyz = {0.0, 0.0, 0.0};
+    size_t poff = 0, f = 0;
+    for (size_t s = 0; s < b.l.size(); ++s) {
+        const int np = b.nprim[s], nc = (b.l[s] + 1) * (b.l[s] + 2) / 2;
+        if (b.atom[s] == a) {
+            o.l.push_back(b.l[s]);
+            o.nprim.push_back(np);
+            o.atom.push_back(0);
+            for (int q = 0; q < 3; ++q) o.center.push_back(0.0);
+            for (int k = 0; k < np; ++k) {
+                o.exponent.push_back(b.exponent[poff + k]);
+                o.coef.push_back(b.coef[poff + k]);
+                o.prim_norm.push_back(b.prim_norm[poff + k]);
+            }
+            for (int k = 0; k < nc; ++k) o.fnorm.push_back(b.fnorm[f + k]);
+            o.nb += nc;
+        }
+        poff += np;
+        f += nc;
+    }
+    return o;
+}
+
+void jacobi_eigh(int n, std::vector<double> A, std::vector<double> &w, std::vector<double> &V) {
+    V.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) V[(size_t)i * n + i] = 1.0;
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0, tot = 0.0;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) {
+                const double v = A[(size_t)i * n + j] * A[(size_t)i * n + j];
+                tot += v;
+                if (i != j) off += v;
+            }
+        if (off <= 1e-30 * tot || off == 0.0) break;
+        for (int p = 0; p < n; ++p)
+            for (int q = p + 1; q < n; ++q) {
+                const double apq = A[(size_t)p * n + q];
+                if (apq == 0.0) continue;
+                const double app = A[(size_t)p * n + p], aqq = A[(size_t)q * n + q];
+                const double theta = (aqq - app) / (2.0 * apq);
+                const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < n; ++k) {  // columns p, q
+                    const double akp = A[(size_t)k * n + p], akq = A[(size_t)k * n + q];
+                    A[(size_t)k * n + p] = c * akp - s * akq;
+                    A[(size_t)k * n + q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < n; ++k) {  // rows p, q
+                    const double apk = A[(size_t)p * n + k], aqk = A[(size_t)q * n + k];
+                    A[(size_t)p * n + k] = c * apk - s * aqk;
+                    A[(size_t)q * n + k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < n; ++k) {
+                    const double vkp = V[(size_t)k * n + p], vkq = V[(size_t)k * n + q];
+                    V[(size_t)k * n + p] = c * vkp - s * vkq;
+                    V[(size_t)k * n + q] = s * vkp + c * vkq;
+                }
+            }
+    }
+    std::vector<int> ord(n);
+    for (int i = 0; i < n; ++i) ord[i] = i;
+    std::sort(ord.begin(), ord.end(), [&](int x, int y) { return A[(size_t)x * n + x] < A[(size_t)y * n + y]; });
+    w.resize(n);
+    std::vector<double> Vs((size_t)n * n);
+    for (int k = 0; k < n; ++k) {
+        w[k] = A[(size_t)ord[k] * n + ord[k]];
+        for (int i = 0; i < n; ++i) Vs[(size_t)i * n + k] = V[(size_t)i * n + ord[k]];
+    }
+    V.swap(Vs);
+}
+
+}  // namespace pkh
+
 extern "C" {
 
 struct pk_basis {

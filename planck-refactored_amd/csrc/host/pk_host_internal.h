@@ -24,6 +24,11 @@ int fail(int code, const std::string &m);
 int load_basis(const char *path, int natoms, const int32_t *Z, const double *xyz, Basis &b);
 double nuclear_repulsion(const Basis &b);
 int one_electron(const Basis &b, double *S, double *T, double *V);
+// the shells of atom a moved to the origin (a one-atom basis of the same element)
+Basis atom_basis(const Basis &b, int a);
+// symmetric eigenproblem by cyclic Jacobi rotations (small matrices):
+// A (n x n, row-major, symmetric) -> w ascending, V columns (row-major V[i*n+k])
+void jacobi_eigh(int n, std::vector<double> A, std::vector<double> &w, std::vector<double> &V);
 
 }  // namespace pkh
 

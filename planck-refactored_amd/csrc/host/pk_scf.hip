@@ -17,6 +17,7 @@
 #include <chrono>
 #include <cmath>
 #include <deque>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -173,6 +174,209 @@ struct Dev {
         }                                                                                   \
     } while (0)
 
+
+// ---------------------------------------------------------------- SAD guess
+// compute_sad_guess_rhf (src/scf/sad.cpp:589-643): atomic UHF per element,
+// spherical average per shell, block-diagonal assembly; the projection on an
+// RHF density runs on the device in pk_scf_rhf.
+
+using Mat = std::vector<double>;  // row-major n x n
+
+Mat mm(int n, const Mat &A, const Mat &B, bool ta = false, bool tb = false) {
+    Mat C((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < n; ++k) {
+            const double a = ta ? A[(size_t)k * n + i] : A[(size_t)i * n + k];
+            if (a == 0.0) continue;
+            for (int j = 0; j < n; ++j) C[(size_t)i * n + j] += a * (tb ? B[(size_t)j * n + k] : B[(size_t)k * n + j]);
+        }
+    return C;
+}
+
+// symmetric f(A) = U diag(f(w)) U^T through the Jacobi eigensolver
+template <typename Fn>
+Mat sym_fun(int n, const Mat &A, Fn f) {
+    std::vector<double> w, V;
+    pkh::jacobi_eigh(n, A, w, V);
+    Mat R((size_t)n * n, 0.0);
+    for (int k = 0; k < n; ++k) {
+        const double fk = f(w[k]);
+        if (fk == 0.0) continue;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) R[(size_t)i * n + j] += V[(size_t)i * n + k] * fk * V[(size_t)j * n + k];
+    }
+    return R;
+}
+
+// atomic UHF (sad.cpp:284-356 through scf.cpp:467-830): multiplicity 1 for even
+// Z, 2 for odd Z; core guess per spin; DIIS per spin (6 vectors, restart when
+// the error grows 2x after iteration 2); converged when iteration > 1 and |dE|,
+// rms and max change of Pa + Pb are below 1e-10.  J and K from the engine.
+int atomic_uhf(const pkh::Basis &ab, int engine, double tol_eri, Mat &Pt, Mat &S) {
+    const int n = ab.nb;
+    const size_t n2 = (size_t)n * n;
+    S.assign(n2, 0.0);
+    Mat T(n2), V(n2), H(n2);
+    pkh::one_electron(ab, S.data(), T.data(), V.data());
+    for (size_t i = 0; i < n2; ++i) H[i] = T[i] + V[i];
+    const int Z = ab.Z[0], unp = Z % 2, na = (Z + unp) / 2, nbeta = (Z - unp) / 2;
+    bool singular = false;
+    const Mat X = sym_fun(n, S, [&](double w) {
+        if (w < 1e-8) singular = true;
+        return w > 0 ? 1.0 / std::sqrt(w) : 0.0;
+    });
+    if (singular) return pkh::fail(PK_ERR_ARG, "SAD: atomic overlap matrix is near-singular");
+    auto dens = [&](const Mat &F, int nocc) {
+        std::vector<double> w, U;
+        pkh::jacobi_eigh(n, mm(n, mm(n, X, F), X), w, U);
+        const Mat C = mm(n, X, U);
+        Mat P(n2, 0.0);
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) {
+                double s = 0;
+                for (int k = 0; k < nocc; ++k) s += C[(size_t)i * n + k] * C[(size_t)j * n + k];
+                P[(size_t)i * n + j] = s;
+            }
+        return P;
+    };
+    Mat Pa = dens(H, na), Pb = dens(H, nbeta);
+    pk_basis_desc d{(int32_t)ab.l.size(), ab.center.data(), ab.l.data(), ab.nprim.data(), ab.exponent.data(),
+                    ab.coef.data(), ab.prim_norm.data()};
+    pk_ctx *ctx = nullptr;
+    if (pk_ctx_create(&d, &ctx) != PK_OK) return pkh::fail(PK_ERR_HIP, std::string("SAD: ") + pk_last_error());
+    int rc = PK_OK;
+    if (pk_ctx_set(ctx, engine, tol_eri, PK_PREC_FP64, 0.0) != PK_OK) rc = pkh::fail(PK_ERR_HIP, pk_last_error());
+    std::deque<std::pair<Mat, Mat>> hist[2];  // (F, e) per spin
+    double Eprev = 0.0, err_prev = 1e300;
+    bool conv = false;
+    Mat Ja(n2), Ka(n2), Jb(n2), Kb(n2);
+    for (int it = 1; it <= 200 && rc == PK_OK && !conv; ++it) {
+        if (pk_fock_jk(ctx, Pa.data(), Ja.data(), Ka.data(), PK_MEM_HOST) != PK_OK ||
+            pk_fock_jk(ctx, Pb.data(), Jb.data(), Kb.data(), PK_MEM_HOST) != PK_OK) {
+            rc = pkh::fail(PK_ERR_HIP, pk_last_error());
+            break;
+        }
+        Mat F[2] = {Mat(n2), Mat(n2)};
+        double E = 0.0;
+        for (size_t i = 0; i < n2; ++i) {
+            F[0][i] = H[i] + Ja[i] + Jb[i] - Ka[i];
+            F[1][i] = H[i] + Ja[i] + Jb[i] - Kb[i];
+            E += 0.5 * (Pa[i] * (H[i] + F[0][i]) + Pb[i] * (H[i] + F[1][i]));
+        }
+        const Mat *Ps[2] = {&Pa, &Pb};
+        Mat e[2];
+        double cur = 0.0;
+        for (int sp = 0; sp < 2; ++sp) {
+            const Mat FPS = mm(n, mm(n, F[sp], *Ps[sp]), S), SPF = mm(n, mm(n, S, *Ps[sp]), F[sp]);
+            Mat D(n2);
+            for (size_t i = 0; i < n2; ++i) D[i] = FPS[i] - SPF[i];
+            e[sp] = mm(n, mm(n, X, D), X);
+            double ss = 0;
+            for (double v : e[sp]) ss += v * v;
+            cur = std::max(cur, std::sqrt(ss / (double)n2));
+        }
+        if (it > 2 && cur > 2.0 * err_prev) hist[0].clear(), hist[1].clear();
+        err_prev = cur;
+        Mat Pn[2];
+        for (int sp = 0; sp < 2; ++sp) {
+            hist[sp].emplace_back(F[sp], e[sp]);
+            if (hist[sp].size() > 6) hist[sp].pop_front();
+            Mat Fd = F[sp];
+            const int m = (int)hist[sp].size();
+            if (m >= 2) {
+                std::vector<double> A((size_t)(m + 1) * (m + 1), 0.0), rhs(m + 1, 0.0);
+                for (int a = 0; a < m; ++a) {
+                    for (int b2 = 0; b2 < m; ++b2) {
+                        double v = 0;
+                        for (size_t k = 0; k < n2; ++k) v += hist[sp][a].second[k] * hist[sp][b2].second[k];
+                        A[(size_t)a * (m + 1) + b2] = v;
+                    }
+                    A[(size_t)a * (m + 1) + m] = A[(size_t)m * (m + 1) + a] = -1.0;
+                }
+                rhs[m] = -1.0;
+                const std::vector<double> c = qr_solve(m + 1, A, rhs);
+                std::fill(Fd.begin(), Fd.end(), 0.0);
+                for (int a = 0; a < m; ++a)
+                    for (size_t k = 0; k < n2; ++k) Fd[k] += c[a] * hist[sp][a].first[k];
+            }
+            Pn[sp] = dens(Fd, sp == 0 ? na : nbeta);
+        }
+        double dmax = 0, dss = 0;
+        for (size_t i = 0; i < n2; ++i) {
+            const double dd = (Pn[0][i] + Pn[1][i]) - (Pa[i] + Pb[i]);
+            dmax = std::max(dmax, std::fabs(dd));
+            dss += dd * dd;
+        }
+        Pa.swap(Pn[0]);
+        Pb.swap(Pn[1]);
+        const double dE = std::fabs(E - Eprev);
+        Eprev = E;
+        conv = it > 1 && dE < 1e-10 && std::sqrt(dss / (double)n2) < 1e-10 && dmax < 1e-10;
+    }
+    pk_ctx_destroy(ctx);
+    if (rc != PK_OK) return rc;
+    if (!conv) return pkh::fail(PK_ERR_ARG, "SAD: atomic UHF for Z=" + std::to_string(Z) + " did not converge");
+    Pt.assign(n2, 0.0);
+    for (size_t i = 0; i < n2; ++i) Pt[i] = Pa[i] + Pb[i];
+    return PK_OK;
+}
+
+// spherical average (sad.cpp:367-430): Mulliken shell populations, cross-shell
+// blocks zeroed, each shell block (N_l / g) pinv(S_shell, 1e-10)
+Mat spherical_average(const pkh::Basis &ab, const Mat &P, const Mat &S) {
+    const int n = ab.nb;
+    const Mat PS = mm(n, P, S);
+    Mat out((size_t)n * n, 0.0);
+    int f0 = 0;
+    for (size_t s = 0; s < ab.l.size(); ++s) {
+        const int g = (ab.l[s] + 1) * (ab.l[s] + 2) / 2;
+        double N = 0;
+        for (int i = 0; i < g; ++i) N += PS[(size_t)(f0 + i) * n + f0 + i];
+        Mat Sb((size_t)g * g);
+        for (int i = 0; i < g; ++i)
+            for (int j = 0; j < g; ++j) Sb[(size_t)i * g + j] = S[(size_t)(f0 + i) * n + f0 + j];
+        const Mat Pinv = sym_fun(g, Sb, [](double w) { return w > 1e-10 ? 1.0 / w : 0.0; });
+        for (int i = 0; i < g; ++i)
+            for (int j = 0; j < g; ++j) out[(size_t)(f0 + i) * n + f0 + j] = N / g * Pinv[(size_t)i * g + j];
+        f0 += g;
+    }
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < i; ++j) {
+            const double v = 0.5 * (out[(size_t)i * n + j] + out[(size_t)j * n + i]);
+            out[(size_t)i * n + j] = out[(size_t)j * n + i] = v;
+        }
+    return out;
+}
+
+// block-diagonal raw SAD density in molecular AO order (sad.cpp:436-470)
+int sad_raw(const pkh::Basis &b, int engine, double tol_eri, Mat &P) {
+    const int n = b.nb, natoms = (int)b.Z.size();
+    std::vector<std::vector<int>> ao(natoms);
+    int f = 0;
+    for (size_t s = 0; s < b.l.size(); ++s) {
+        const int g = (b.l[s] + 1) * (b.l[s] + 2) / 2;
+        for (int i = 0; i < g; ++i) ao[b.atom[s]].push_back(f + i);
+        f += g;
+    }
+    std::map<int, Mat> cache;
+    P.assign((size_t)n * n, 0.0);
+    for (int a = 0; a < natoms; ++a) {
+        const int z = b.Z[a];
+        if (!cache.count(z)) {
+            const pkh::Basis ab = pkh::atom_basis(b, a);
+            Mat Pt, Sa;
+            const int rc = atomic_uhf(ab, engine, tol_eri, Pt, Sa);
+            if (rc != PK_OK) return rc;
+            cache[z] = spherical_average(ab, Pt, Sa);
+        }
+        const Mat &B = cache[z];
+        const int g = (int)ao[a].size();
+        for (int i = 0; i < g; ++i)
+            for (int j = 0; j < g; ++j) P[(size_t)ao[a][i] * n + ao[a][j]] = B[(size_t)i * g + j];
+    }
+    return PK_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -190,6 +394,8 @@ void pk_scf_default_options(pk_scf_options *o) {
     o->precision = PK_PREC_FP64;
     o->fp32_cutoff = 0.0;
     o->guess_fallback = 0;
+    o->sad_guess = 0;
+    o->guess_only = 0;
 }
 
 int pk_scf_rhf(const pk_basis *bas, const pk_scf_options *opt, pk_scf_result *res, double *P_out, double *hist,
@@ -292,7 +498,29 @@ int pk_scf_rhf(const pk_basis *bas, const pk_scf_options *opt, pk_scf_result *re
     if (opt->use_guess && opt->guess_density) {
         SCF_HIP(hipMemcpyAsync(P, opt->guess_density, n2 * 8, hipMemcpyHostToDevice, st));
         res->guess_used = 1;
+    } else if (opt->sad_guess) {
+        // SAD: raw block-diagonal density on the host, then the projection on
+        // an RHF density (sad.cpp:476-505): the n_occ eigenvectors of X P X
+        // with the largest eigenvalues, P = 2 C C^T with C = X U
+        {
+            Mat Praw;
+            if ((rc = sad_raw(b, opt->engine, opt->tol_eri, Praw)) != PK_OK) goto done;
+            SCF_HIP(hipMemcpyAsync(D, Praw.data(), n2 * 8, hipMemcpyHostToDevice, st));
+            SCF_HIP(hipStreamSynchronize(st));
+        }
+        SCF_RB(gemm(rocblas_operation_none, rocblas_operation_none, n, 1.0, X, D, M));
+        SCF_RB(gemm(rocblas_operation_none, rocblas_operation_none, n, 1.0, M, X, W1));
+        SCF_RB(rocsolver_dsyevd(hb, rocblas_evect_original, rocblas_fill_upper, n, W1, n, ev, ee, info));
+        SCF_RB(gemm(rocblas_operation_none, rocblas_operation_none, n, 1.0, X, W1, C));
+        SCF_RB(rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_transpose, n, n, nocc, &two,
+                             C + (size_t)(n - nocc) * n, n, C + (size_t)(n - nocc) * n, n, &zero, P, n));
     } else if ((rc = density(H, P)) != PK_OK) {
+        goto done;
+    }
+    if (opt->guess_only) {
+        if (P_out) SCF_HIP(hipMemcpyAsync(P_out, P, n2 * 8, hipMemcpyDeviceToHost, st));
+        SCF_HIP(hipStreamSynchronize(st));
+        res->converged = 1;
         goto done;
     }
     if (opt->use_guess && opt->guess_density && opt->guess_fallback) {

@@ -23,7 +23,8 @@ class ScfOptions(ctypes.Structure):
                 ("diis_dim", ctypes.c_int32), ("max_cycles", ctypes.c_int32), ("engine", ctypes.c_int32),
                 ("charge", ctypes.c_int32), ("use_guess", ctypes.c_int32),
                 ("guess_density", ctypes.POINTER(ctypes.c_double)), ("precision", ctypes.c_int32),
-                ("fp32_cutoff", ctypes.c_double), ("guess_fallback", ctypes.c_int32)]
+                ("fp32_cutoff", ctypes.c_double), ("guess_fallback", ctypes.c_int32), ("sad_guess", ctypes.c_int32),
+                ("guess_only", ctypes.c_int32)]
 
 
 class ScfResult(ctypes.Structure):
@@ -112,7 +113,7 @@ class Molecule:
         return _eng.FockEngine(self.shells(), engine=engine, tol_eri=tol_eri)
 
     def rhf(self, engine="os", tol_eri=1e-10, tol_energy=1e-10, tol_density=1e-10, diis_dim=8, max_cycles=0,
-            guess=None, fp32_cutoff=None, guess_fallback=False):
+            guess=None, fp32_cutoff=None, guess_fallback=False, sad=False, guess_only=False):
         """Device RHF (pk_scf_rhf).  fp32_cutoff: mixed precision, quartets
         with Schwarz bound below it in FP32 (None: FP64 throughout)."""
         lib = load()
@@ -121,6 +122,7 @@ class Molecule:
         o.tol_energy, o.tol_density, o.tol_eri = tol_energy, tol_density, tol_eri
         if fp32_cutoff is not None:
             o.precision, o.fp32_cutoff = _eng.PREC_MIXED, float(fp32_cutoff)
+        o.sad_guess, o.guess_only = int(bool(sad)), int(bool(guess_only))
         o.diis_dim, o.max_cycles = diis_dim, max_cycles
         o.engine = _eng.ENGINE[engine] if isinstance(engine, str) else int(engine)
         o.charge = self.charge

@@ -5,11 +5,13 @@ Bar: total energy within 1e-9 Eh of the reference-held value.
 """
 import json
 import os
+import sys
 
 import numpy as np
 import pytest
 
 from pkfock.host import Molecule
+from pkfock.molecules import ANGSTROM_TO_BOHR, water
 
 pytestmark = pytest.mark.gpu
 
@@ -29,3 +31,62 @@ def test_reference_energy(case):
     assert abs(r["energy"] - case["energy"]) < 1e-9
     if "iterations" in case:
         assert abs(r["iterations"] - case["iterations"]) <= 2
+
+
+# ---------------------------------------------------------------- SAD guess
+sys.path.insert(0, os.path.join(os.path.dirname(HERE), "oracle"))
+import sad_ref  # noqa: E402
+
+_ETHYLENE_CCPVDZ = ([6, 6, 1, 1, 1, 1], np.array([[-3.438338, 0.906062, 0.0], [-3.438338, 2.230667, -0.000001],
+                                                  [-2.836048, 0.343683, -0.709301], [-4.040627, 0.343684, 0.709301],
+                                                  [-2.836049, 2.793045, -0.709302], [-4.040627, 2.793046, 0.709300]]))
+
+
+def _sad_oracle(basis, Z, X):
+    from test_oracle import sad_parts
+    ref, Praw = sad_parts(basis, Z, np.asarray(X) * ANGSTROM_TO_BOHR)
+    return sad_ref.project(Praw, ref.matrix("overlap"), int(sum(Z)))[0]
+
+
+@pytest.mark.parametrize("case", ["water", "water3", "ethylene"])
+def test_sad_density_matches_oracle(case):
+    """Device SCF's SAD initial density (atomic UHF on the engine, host
+    averaging, device projection) against the oracle's restatement; the cases
+    have non-degenerate natural occupations at n_occ, so the projection is
+    unique.  Bar 1e-6: both atomic UHFs stop when the density change per
+    iteration is below 1e-10, along different DIIS / eigensolver paths (the
+    carbon atom's occupied p orbital is a choice inside a degenerate shell);
+    measured 2e-9 (water) to 7e-8 (ethylene cc-pVDZ)."""
+    from pkfock.molecules import water_cluster
+    if case == "water":
+        (Z, X), basis = water(), "6-31g*"
+    elif case == "water3":
+        (Z, X), basis = water_cluster(7, n=3), "6-31g*"
+    else:
+        (Z, X), basis = _ETHYLENE_CCPVDZ, "cc-pvdz"
+    mol = Molecule(Z, X, basis)
+    r = mol.rhf(tol_eri=1e-12, sad=True, guess_only=True)
+    P_ref = _sad_oracle(basis, Z, X)
+    d = np.abs(r["P"] - P_ref).max()
+    print("%s/%s SAD max|dP| = %.2e" % (case, basis, d))
+    assert d < 1e-6
+
+
+def test_scf_from_sad_reaches_reference_energies():
+    """SCF from the SAD guess reaches the reference energies where the SAD
+    projection is unique: ethylene cc-pVDZ (regression_cases.json,
+    -78.040190575) and water 6-31G* (the core-guess pin of this file).  Not
+    asserted: ethylene STO-3G at the engine_symmetry geometry, whose raw SAD
+    density has a degenerate natural-occupation pair exactly at n_occ, so the
+    projected guess (and the SCF solution reached from it) depends on the
+    eigensolver, in the reference as here."""
+    Z, X = _ETHYLENE_CCPVDZ
+    r = Molecule(Z, X, "cc-pvdz").rhf(engine="auto", tol_eri=1e-10, sad=True, max_cycles=100)
+    print("ethylene cc-pVDZ from SAD: %.10f in %d iterations" % (r["energy"], r["iterations"]))
+    assert r["rc"] == 0 and abs(r["energy"] - (-78.040190575)) < 1e-9
+    Z, X = water()
+    core = Molecule(Z, X, "6-31g*").rhf(tol_eri=1e-12)
+    r = Molecule(Z, X, "6-31g*").rhf(tol_eri=1e-12, sad=True)
+    print("water 6-31G* from SAD: %.10f in %d iterations (core guess: %d)" % (r["energy"], r["iterations"],
+                                                                            core["iterations"]))
+    assert r["rc"] == 0 and abs(r["energy"] - core["energy"]) < 1e-9
