@@ -644,4 +644,236 @@ done:
     return rc;
 }
 
+// Batched RHF: every molecule runs the iteration of pk_scf_rhf, but the Fock
+// builds of one iteration are one batched build of all molecules (one context,
+// merged per-class quartet lists, hipGraph replay); the per-molecule algebra
+// (energy, DIIS, diagonalisation) follows on the same stream.  Converged
+// molecules keep their density and stop updating.
+int pk_scf_rhf_batch(const pk_basis *const *bases, int nmol, const pk_scf_options *opt, pk_scf_result *res,
+                     double *const *P_out, const double *const *guess) {
+    if (!bases || nmol <= 0 || !opt || !res) return pkh::fail(PK_ERR_ARG, "pk_scf_rhf_batch: null argument");
+    const auto t_start = std::chrono::steady_clock::now();
+    struct Mol {
+        const pkh::Basis *b;
+        int n = 0, nocc = 0;
+        size_t n2 = 0;
+        double *S, *H, *X, *Pn, *F, *M, *D, *W1, *C, *ev, *ee, *part, *P = nullptr, *G = nullptr;
+        rocblas_int *info;
+        std::vector<double *> Fh, Eh;
+        std::deque<int> slots;
+        std::vector<std::vector<double>> Bm;
+        double Eprev = 0.0;
+        bool done = false;
+    };
+    std::vector<Mol> mols(nmol);
+    Dev dev;
+    hipStream_t st = nullptr;
+    rocblas_handle hb = nullptr;
+    pk_ctx *ctx = nullptr;
+    int rc = PK_OK;
+    const double one = 1.0, zero = 0.0, two = 2.0;
+    const int nparts = 256;
+    const int diis_dim = opt->diis_dim > 0 ? opt->diis_dim : 8;
+    int max_cycles = opt->max_cycles, auto_cycles = 0;
+    std::vector<pk_basis_desc> descs(nmol);
+    std::vector<int32_t> lay_nb(nmol);
+    std::vector<int64_t> lay_off(nmol);
+    size_t ntot = 0;
+    double *Pp = nullptr, *Gp = nullptr;
+    std::vector<double> hpart(2 * nparts);
+    (void)one;
+    auto gemm = [&](Mol &m, rocblas_operation ta, rocblas_operation tb, int k, const double *A, const double *B, double *Cm) {
+        return rocblas_dgemm(hb, ta, tb, m.n, m.n, k, &one, A, m.n, B, m.n, &zero, Cm, m.n);
+    };
+    auto density = [&](Mol &m, const double *Fm, double *Pt) -> int {
+        const int n = m.n;
+        if (gemm(m, rocblas_operation_none, rocblas_operation_none, n, m.X, Fm, m.M) != rocblas_status_success ||
+            gemm(m, rocblas_operation_none, rocblas_operation_none, n, m.M, m.X, m.W1) != rocblas_status_success ||
+            rocsolver_dsyevd(hb, rocblas_evect_original, rocblas_fill_upper, n, m.W1, n, m.ev, m.ee, m.info) !=
+                rocblas_status_success ||
+            gemm(m, rocblas_operation_none, rocblas_operation_none, n, m.X, m.W1, m.C) != rocblas_status_success ||
+            rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_transpose, n, n, m.nocc, &two, m.C, n, m.C, n,
+                          &zero, Pt, n) != rocblas_status_success)
+            return pkh::fail(PK_ERR_HIP, "pk_scf_rhf_batch: dense algebra failed");
+        return PK_OK;
+    };
+
+    SCF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    SCF_RB(rocblas_create_handle(&hb));
+    SCF_RB(rocblas_set_stream(hb, st));
+    for (int i = 0; i < nmol; ++i) {
+        Mol &m = mols[i];
+        if (!bases[i]) {
+            rc = pkh::fail(PK_ERR_ARG, "pk_scf_rhf_batch: null basis");
+            goto done;
+        }
+        m.b = &pkh_basis_of(bases[i]);
+        m.n = m.b->nb;
+        m.n2 = (size_t)m.n * m.n;
+        int nel = -opt->charge;
+        for (int z : m.b->Z) nel += z;
+        if (nel % 2 != 0) {
+            rc = pkh::fail(PK_ERR_ARG, "RHF requires an even number of electrons (closed shell)");
+            goto done;
+        }
+        m.nocc = nel / 2;
+        pk_basis_describe(bases[i], &descs[i]);
+        res[i] = pk_scf_result{};
+        res[i].nuclear_repulsion = pkh::nuclear_repulsion(*m.b);
+        const size_t n2 = m.n2;
+        for (double **b : {&m.S, &m.H, &m.X, &m.Pn, &m.F, &m.M, &m.D, &m.W1, &m.C}) *b = dev.alloc(n2);
+        m.ev = dev.alloc(m.n), m.ee = dev.alloc(m.n), m.part = dev.alloc(2 * nparts);
+        m.info = (rocblas_int *)dev.alloc(1);
+        for (int s = 0; s < diis_dim; ++s) {
+            m.Fh.push_back(dev.alloc(n2));
+            m.Eh.push_back(dev.alloc(n2));
+        }
+        if (!m.S || !m.H || !m.X || !m.Pn || !m.F || !m.M || !m.D || !m.W1 || !m.C || !m.ev || !m.ee || !m.part ||
+            !m.info || !m.Fh.back() || !m.Eh.back()) {
+            rc = pkh::fail(PK_ERR_ALLOC, "pk_scf_rhf_batch: device allocation failed");
+            goto done;
+        }
+        m.Bm.assign(diis_dim, std::vector<double>(diis_dim, 0.0));
+        auto_cycles = std::max(auto_cycles, m.n > 1000 ? 300 : m.n > 500 ? 200 : m.n > 250 ? 100 : 50);
+    }
+    if (max_cycles <= 0) max_cycles = auto_cycles;
+    // the batch context: one build per iteration for all molecules
+    SCF_PK(pk_batch_create(descs.data(), nmol, &ctx));
+    SCF_PK(pk_ctx_set_stream(ctx, st));
+    SCF_PK(pk_ctx_set(ctx, opt->engine, opt->tol_eri, opt->precision, opt->fp32_cutoff));
+    SCF_PK(pk_ctx_set_graph(ctx, 1));
+    SCF_PK(pk_ctx_mol_layout(ctx, lay_nb.data(), lay_off.data()));
+    ntot = (size_t)(lay_off[nmol - 1] + (int64_t)lay_nb[nmol - 1] * lay_nb[nmol - 1]);
+    Pp = dev.alloc(ntot), Gp = dev.alloc(ntot);
+    if (!Pp || !Gp) {
+        rc = pkh::fail(PK_ERR_ALLOC, "pk_scf_rhf_batch: device allocation failed");
+        goto done;
+    }
+    for (int i = 0; i < nmol; ++i) {
+        Mol &m = mols[i];
+        const size_t n2 = m.n2;
+        const int n = m.n, g2 = (int)((n2 + 255) / 256);
+        m.P = Pp + lay_off[i];
+        m.G = Gp + lay_off[i];
+        {
+            std::vector<double> hS(n2), hT(n2), hV(n2), w(n);
+            pkh::one_electron(*m.b, hS.data(), hT.data(), hV.data());
+            for (size_t k = 0; k < n2; ++k) hT[k] += hV[k];
+            SCF_HIP(hipMemcpyAsync(m.S, hS.data(), n2 * 8, hipMemcpyHostToDevice, st));
+            SCF_HIP(hipMemcpyAsync(m.H, hT.data(), n2 * 8, hipMemcpyHostToDevice, st));
+            SCF_HIP(hipMemcpyAsync(m.W1, m.S, n2 * 8, hipMemcpyDeviceToDevice, st));
+            SCF_RB(rocsolver_dsyevd(hb, rocblas_evect_original, rocblas_fill_upper, n, m.W1, n, m.ev, m.ee, m.info));
+            SCF_HIP(hipMemcpyAsync(w.data(), m.ev, n * 8, hipMemcpyDeviceToHost, st));
+            SCF_HIP(hipStreamSynchronize(st));
+            if (w[0] < 1e-8) {
+                rc = pkh::fail(PK_ERR_ARG, "Overlap matrix is near-singular in molecule " + std::to_string(i));
+                goto done;
+            }
+        }
+        SCF_HIP(hipMemcpyAsync(m.C, m.W1, n2 * 8, hipMemcpyDeviceToDevice, st));
+        hipLaunchKernelGGL(k_scale_cols, dim3(g2), dim3(256), 0, st, m.C, m.ev, n);
+        SCF_RB(gemm(m, rocblas_operation_none, rocblas_operation_transpose, n, m.C, m.W1, m.X));
+        if (guess && guess[i]) {
+            SCF_HIP(hipMemcpyAsync(m.P, guess[i], n2 * 8, hipMemcpyHostToDevice, st));
+            res[i].guess_used = 1;
+        } else if ((rc = density(m, m.H, m.P)) != PK_OK) {
+            goto done;
+        }
+    }
+    for (int it = 1; it <= max_cycles; ++it) {
+        SCF_PK(pk_fock_2e(ctx, Pp, Gp, PK_MEM_DEVICE));
+        double build_ms = 0;
+        {
+            pk_stats s;
+            pk_ctx_stats(ctx, &s);
+            build_ms = s.last_build_ms;
+        }
+        bool all = true;
+        for (int i = 0; i < nmol; ++i) {
+            Mol &m = mols[i];
+            if (m.done) continue;
+            const int n = m.n, g2 = (int)((m.n2 + 255) / 256);
+            const size_t n2 = m.n2;
+            res[i].fock_ms += build_ms;
+            hipLaunchKernelGGL(k_add, dim3(g2), dim3(256), 0, st, m.H, m.G, m.F, n2);
+            double eph = 0, epf = 0;
+            SCF_RB(rocblas_set_pointer_mode(hb, rocblas_pointer_mode_host));
+            SCF_RB(rocblas_ddot(hb, (rocblas_int)n2, m.P, 1, m.H, 1, &eph));
+            SCF_RB(rocblas_ddot(hb, (rocblas_int)n2, m.P, 1, m.F, 1, &epf));
+            const double Etot = 0.5 * (eph + epf) + res[i].nuclear_repulsion;
+            SCF_RB(gemm(m, rocblas_operation_none, rocblas_operation_none, n, m.F, m.P, m.W1));
+            SCF_RB(gemm(m, rocblas_operation_none, rocblas_operation_none, n, m.W1, m.S, m.M));
+            hipLaunchKernelGGL(k_antisym, dim3(g2), dim3(256), 0, st, m.M, m.D, n);
+            SCF_RB(gemm(m, rocblas_operation_none, rocblas_operation_none, n, m.X, m.D, m.W1));
+            int slot;
+            if ((int)m.slots.size() < diis_dim) {
+                slot = (int)m.slots.size();
+            } else {
+                slot = m.slots.front();
+                m.slots.pop_front();
+            }
+            SCF_RB(gemm(m, rocblas_operation_none, rocblas_operation_none, n, m.W1, m.X, m.Eh[slot]));
+            SCF_HIP(hipMemcpyAsync(m.Fh[slot], m.F, n2 * 8, hipMemcpyDeviceToDevice, st));
+            m.slots.push_back(slot);
+            for (int s2 : m.slots) {
+                double v = 0;
+                SCF_RB(rocblas_ddot(hb, (rocblas_int)n2, m.Eh[slot], 1, m.Eh[s2], 1, &v));
+                m.Bm[slot][s2] = m.Bm[s2][slot] = v;
+            }
+            if (it == 1) res[i].first_diis_error = std::sqrt(m.Bm[slot][slot] / (double)n2);
+            const double *Fd = m.F;
+            if (m.slots.size() >= 2) {
+                const int k = (int)m.slots.size();
+                std::vector<double> A((size_t)(k + 1) * (k + 1), 0.0), rhs(k + 1, 0.0);
+                for (int a = 0; a < k; ++a) {
+                    for (int b2 = 0; b2 < k; ++b2) A[(size_t)a * (k + 1) + b2] = m.Bm[m.slots[a]][m.slots[b2]];
+                    A[(size_t)a * (k + 1) + k] = A[(size_t)k * (k + 1) + a] = -1.0;
+                }
+                rhs[k] = -1.0;
+                const std::vector<double> cf = qr_solve(k + 1, A, rhs);
+                SCF_HIP(hipMemsetAsync(m.M, 0, n2 * 8, st));
+                for (int a = 0; a < k; ++a) SCF_RB(rocblas_daxpy(hb, (rocblas_int)n2, &cf[a], m.Fh[m.slots[a]], 1, m.M, 1));
+                SCF_HIP(hipMemcpyAsync(m.D, m.M, n2 * 8, hipMemcpyDeviceToDevice, st));
+                Fd = m.D;
+            }
+            if ((rc = density(m, Fd, m.Pn)) != PK_OK) goto done;
+            hipLaunchKernelGGL(k_diff, dim3(nparts), dim3(256), 0, st, m.Pn, m.P, n2, m.part);
+            SCF_HIP(hipMemcpyAsync(hpart.data(), m.part, 2 * nparts * 8, hipMemcpyDeviceToHost, st));
+            SCF_HIP(hipMemcpyAsync(m.P, m.Pn, n2 * 8, hipMemcpyDeviceToDevice, st));
+            SCF_HIP(hipStreamSynchronize(st));
+            double dmax = 0, dss = 0;
+            for (int k = 0; k < nparts; ++k) {
+                dmax = std::max(dmax, hpart[2 * k]);
+                dss += hpart[2 * k + 1];
+            }
+            const double drms = std::sqrt(dss / (double)n2), dE = std::fabs(Etot - m.Eprev);
+            m.Eprev = Etot;
+            res[i].iterations = it;
+            res[i].energy = Etot;
+            if (it > 1 && dE < opt->tol_energy && drms < opt->tol_density && dmax < opt->tol_density) {
+                res[i].converged = 1;
+                m.done = true;
+            }
+            all = all && m.done;
+        }
+        if (all) break;
+    }
+    for (int i = 0; i < nmol; ++i)
+        if (P_out && P_out[i]) SCF_HIP(hipMemcpyAsync(P_out[i], mols[i].P, mols[i].n2 * 8, hipMemcpyDeviceToHost, st));
+    SCF_HIP(hipStreamSynchronize(st));
+    for (int i = 0; i < nmol; ++i)
+        if (!res[i].converged) rc = pkh::fail(1, "SCF did not converge in " + std::to_string(max_cycles) +
+                                                     " iterations for molecule " + std::to_string(i));
+done:
+    if (st) (void)hipStreamSynchronize(st);
+    {
+        const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+        for (int i = 0; i < nmol; ++i) res[i].wall_s = wall;
+    }
+    if (ctx) pk_ctx_destroy(ctx);
+    if (hb) rocblas_destroy_handle(hb);
+    if (st) (void)hipStreamDestroy(st);
+    return rc;
+}
+
 }  // extern "C"

@@ -55,6 +55,7 @@ def load():
     lib.pk_nuclear_repulsion.restype = ctypes.c_double
     lib.pk_scf_default_options.argtypes = [ctypes.POINTER(ScfOptions)]
     lib.pk_scf_rhf.argtypes = [vp, ctypes.POINTER(ScfOptions), ctypes.POINTER(ScfResult), vp, vp, ip]
+    lib.pk_scf_rhf_batch.argtypes = [vp, ip, ctypes.POINTER(ScfOptions), vp, vp, vp]
     lib.pk_host_last_error.restype = ctypes.c_char_p
     _lib = lib
     return lib
@@ -111,6 +112,42 @@ class Molecule:
 
     def engine(self, engine="os", tol_eri=1e-10):
         return _eng.FockEngine(self.shells(), engine=engine, tol_eri=tol_eri)
+
+    @staticmethod
+    def rhf_batch(mols, engine="os", tol_eri=1e-10, tol_energy=1e-10, tol_density=1e-10, diis_dim=8, max_cycles=0,
+                  guesses=None, fp32_cutoff=None):
+        """Batched RHF of independent molecules (pk_scf_rhf_batch): one batched
+        Fock build per iteration for all of them.  Returns a list of result dicts
+        (with "P")."""
+        lib = load()
+        o = ScfOptions()
+        lib.pk_scf_default_options(ctypes.byref(o))
+        o.tol_energy, o.tol_density, o.tol_eri = tol_energy, tol_density, tol_eri
+        o.diis_dim, o.max_cycles = diis_dim, max_cycles
+        o.engine = _eng.ENGINE[engine] if isinstance(engine, str) else int(engine)
+        o.charge = 0
+        if fp32_cutoff is not None:
+            o.precision, o.fp32_cutoff = _eng.PREC_MIXED, float(fp32_cutoff)
+        nm = len(mols)
+        res = (ScfResult * nm)()
+        Ps = [np.zeros((m.nbasis, m.nbasis)) for m in mols]
+        pout = (ctypes.c_void_p * nm)(*[p.ctypes.data for p in Ps])
+        keep = None
+        gin = None
+        if guesses is not None:
+            keep = [None if g is None else np.ascontiguousarray(g, dtype=np.float64) for g in guesses]
+            gin = (ctypes.c_void_p * nm)(*[None if g is None else g.ctypes.data for g in keep])
+        bases = (ctypes.c_void_p * nm)(*[m.handle.value for m in mols])
+        rc = lib.pk_scf_rhf_batch(bases, nm, ctypes.byref(o), res, pout, gin)
+        if rc < 0:
+            raise _eng.PkError("pk_scf_rhf_batch: " + lib.pk_host_last_error().decode())
+        out = []
+        for i in range(nm):
+            d = {k: getattr(res[i], k) for k, _ in ScfResult._fields_}
+            d["P"] = Ps[i]
+            d["rc"] = 0 if res[i].converged else 1
+            out.append(d)
+        return out
 
     def rhf(self, engine="os", tol_eri=1e-10, tol_energy=1e-10, tol_density=1e-10, diis_dim=8, max_cycles=0,
             guess=None, fp32_cutoff=None, guess_fallback=False, sad=False, guess_only=False):
