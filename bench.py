@@ -111,11 +111,32 @@ def bench_water_clusters(args):
     print("bench: water clusters nb_total=%d setup %.2fs; batched %.2f ms/build (graph), one-at-a-time %.2f ms; "
           "max|dJ| batch vs single %.1e" % (eng.nbasis, setup_s, ms, seq_ms, dmax), file=sys.stderr, flush=True)
     mixed = args.fp32_cutoff is not None
+    del singles, dPs, dJs, dKs
+    scf = None
+    if not args.no_scf:
+        mols_h = [Molecule(Z, X, "6-31g*") for Z, X in mols]
+        opts = dict(tol_eri=1e-12, tol_energy=1e-8, tol_density=1e-6)
+        t = time.perf_counter()
+        rb = Molecule.rhf_batch(mols_h, **opts)
+        t_batch = time.perf_counter() - t
+        t = time.perf_counter()
+        rs = [m.rhf(**opts) for m in mols_h]
+        t_seq = time.perf_counter() - t
+        dE = max(abs(a["energy"] - b["energy"]) for a, b in zip(rb, rs))
+        scf = {"batched_wall_s": round(t_batch, 3), "one_at_a_time_wall_s": round(t_seq, 3),
+               "max_iterations": max(r["iterations"] for r in rb), "converged": all(r["rc"] == 0 for r in rb),
+               "max_abs_dE_batch_vs_single": dE}
+        if mixed:
+            t = time.perf_counter()
+            rm = Molecule.rhf_batch(mols_h, fp32_cutoff=args.fp32_cutoff, **opts)
+            scf["mixed_batched_wall_s"] = round(time.perf_counter() - t, 3)
+            scf["max_abs_dE_mixed_vs_fp64"] = max(abs(a["energy"] - b["energy"]) for a, b in zip(rm, rb))
+        print("bench: water-cluster SCF %s" % scf, file=sys.stderr, flush=True)
     peak = PEAK_FP32_TFLOPS if dom["precision"] else PEAK_FP64_TFLOPS
     out = {
         "metric": METRIC, "value": args.steps / elapsed, "unit": "builds/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "strong",
-        "vs_baseline": None, "dtype": "f64+f32" if mixed else "f64", "data": "synthetic",
+        "vs_baseline": None, "dtype": "f64+f32" if mixed else "f64", "data": "synthetic", "scf": scf,
         "config": {"workload": WORKLOADS["water-clusters"][2] + (", mixed FP32 below Schwarz %g" % args.fp32_cutoff
                                                                if mixed else ", FP64"),
                    "molecules": 256, "nbasis_total": eng.nbasis, "shell_quartets": st["quartets"],
