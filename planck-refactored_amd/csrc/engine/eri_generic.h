@@ -76,12 +76,12 @@ __device__ __forceinline__ void rys_root_rt(int n, double T, int r, double &u, d
 // Every lane calls it (the shuffles need the whole wave).
 template <typename Term, typename Emit>
 __device__ __forceinline__ void seg_reduce(int lane, int nout, int nterm, Term term, Emit emit) {
-    int lpo = 1;
-    while (lpo < 64 && lpo * 2 * nout <= 64) lpo *= 2;
-    const int opp = 64 / lpo;
+    int sh = 0;  // lpo = 2^sh lanes per output
+    while (sh < 6 && (2 << sh) * nout <= 64) ++sh;
+    const int lpo = 1 << sh, opp = 64 >> sh;
     const int sl = lane & (lpo - 1);
     for (int o0 = 0; o0 < nout; o0 += opp) {
-        const int o = o0 + lane / lpo;
+        const int o = o0 + (lane >> sh);
         double s = 0.0;
         if (o < nout)
             for (int t = sl; t < nterm; t += lpo) s += term(o, t);
@@ -98,23 +98,65 @@ static __device__ const unsigned char kCartTab[4][10][3] = {
     {{2, 0, 0}, {1, 1, 0}, {1, 0, 1}, {0, 2, 0}, {0, 1, 1}, {0, 0, 2}},
     {{3, 0, 0}, {2, 1, 0}, {2, 0, 1}, {1, 2, 0}, {1, 1, 1}, {1, 0, 2}, {0, 3, 0}, {0, 2, 1}, {0, 1, 2}, {0, 0, 3}}};
 
+// Index tables of one class, built once per workgroup so that the hot loops
+// decode their indices by lookups instead of integer divisions.
+struct GenTabs {
+    unsigned char ab_a[100], ab_b[100], cd_c[100], cd_d[100];  // ab -> (a, b), cd -> (c, d)
+    unsigned char ac_a[100], ac_c[100], bd_b[100], bd_d[100];  // pairs of the exchange sums
+    unsigned char bc_b[100], bc_c[100], ad_a[100], ad_d[100];
+    unsigned short ab_g[100][3];                               // G offset of (a, b) per axis
+    unsigned char cd_g[100][3];                                // G offset of (c, d) per axis
+    unsigned char g_a[256], g_b[256], g_c[256], g_d[256];      // G index -> exponents
+};
+
 __global__ __launch_bounds__(64) void k_generic(GenArgs a) {
     extern __shared__ double smem[];
     __shared__ double su[GEN_NR], sw[GEN_NR];
+    __shared__ GenTabs tb;
     const int lane = threadIdx.x;
     const int item = blockIdx.x;
     if (item >= a.nitems) return;
     const int la = a.la, lb = a.lb, lc = a.lc, ld = a.ld;
     const int NA = (la + 1) * (la + 2) / 2, NB = (lb + 1) * (lb + 2) / 2;
     const int NC = (lc + 1) * (lc + 2) / 2, ND = (ld + 1) * (ld + 2) / 2;
-    const int NCD = NC * ND, NABCD = NA * NB * NCD;
+    const int NAB = NA * NB, NCD = NC * ND, NABCD = NAB * NCD;
     const int E = la + lb, F = lc + ld, n = (E + F) / 2 + 1;
     const int F1 = F + 1, NI = (E + 1) * F1;
-    const int D2 = lb + 1, D3 = lc + 1, D4 = ld + 1;
-    const int NG = (la + 1) * D2 * D3 * D4;  // transferred 1-D factors per (root, axis)
-    double *I2 = smem;                       // [n * 3][E + 1][F + 1]
-    double *G = I2 + n * 3 * NI;             // [n * 3][la + 1][lb + 1][lc + 1][ld + 1]
-    double *acc = G + n * 3 * NG;            // [NABCD]
+    const int D2 = lb + 1, D3 = lc + 1, D4 = ld + 1, D34 = D3 * D4;
+    const int NG = (la + 1) * D2 * D34;  // transferred 1-D factors per (root, axis)
+    double *I2 = smem;                   // [n * 3][E + 1][F + 1]
+    double *G = I2 + n * 3 * NI;         // [n * 3][la + 1][lb + 1][lc + 1][ld + 1]
+    double *acc = G + n * 3 * NG;        // [NABCD]
+
+    for (int i = lane; i < NAB; i += 64) {
+        const int ia = i / NB, ib = i % NB;
+        tb.ab_a[i] = ia;
+        tb.ab_b[i] = ib;
+        for (int q = 0; q < 3; ++q) tb.ab_g[i][q] = (kCartTab[la][ia][q] * D2 + kCartTab[lb][ib][q]) * D34;
+    }
+    for (int i = lane; i < NCD; i += 64) {
+        const int ic = i / ND, id = i % ND;
+        tb.cd_c[i] = ic;
+        tb.cd_d[i] = id;
+        for (int q = 0; q < 3; ++q) tb.cd_g[i][q] = kCartTab[lc][ic][q] * D4 + kCartTab[ld][id][q];
+    }
+    for (int i = lane; i < NA * NC; i += 64) tb.ac_a[i] = i / NC, tb.ac_c[i] = i % NC;
+    for (int i = lane; i < NB * ND; i += 64) tb.bd_b[i] = i / ND, tb.bd_d[i] = i % ND;
+    for (int i = lane; i < NB * NC; i += 64) tb.bc_b[i] = i / NC, tb.bc_c[i] = i % NC;
+    for (int i = lane; i < NA * ND; i += 64) tb.ad_a[i] = i / ND, tb.ad_d[i] = i % ND;
+    for (int g = lane; g < NG; g += 64) {
+        int t = g;
+        tb.g_d[g] = t % D4;
+        t /= D4;
+        tb.g_c[g] = t % D3;
+        t /= D3;
+        tb.g_b[g] = t % D2;
+        tb.g_a[g] = t / D2;
+    }
+    // this lane's first (ab, cd) in the flattened component loops, and the
+    // step of 64 components in (ab, cd)
+    const int ab0 = lane / NCD, cd0 = lane % NCD, sab = 64 / NCD, scd = 64 % NCD;
+    __syncthreads();
 
     int bra, kbeg, kcnt, nb = 0;
     const double *P = nullptr, *Qm = nullptr;
@@ -202,50 +244,41 @@ __global__ __launch_bounds__(64) void k_generic(GenArgs a) {
                 }
                 __syncthreads();
                 // transferred factors G[a][b][c][d] = sum_k sum_l C(b,k) AB^(b-k)
-                // C(d,l) CD^(d-l) I[a+k][c+l], spread over all lanes
-                for (int t = lane; t < n * 3 * NG; t += 64) {
-                    const int ra = t / NG;
-                    int g = t - ra * NG;
-                    const int id = g % D4;
-                    g /= D4;
-                    const int ic = g % D3;
-                    g /= D3;
-                    const int ib = g % D2;
-                    const int ia = g / D2;
+                // C(d,l) CD^(d-l) I[a+k][c+l]; (root, axis) uniform, G index over lanes
+                for (int ra = 0; ra < 3 * n; ++ra) {
                     const int ax = ra % 3;
                     const double ABq = sel3(ax, pb.ab[0], pb.ab[1], pb.ab[2]);
                     const double CDq = sel3(ax, pk.ab[0], pk.ab[1], pk.ab[2]);
                     const double *I = I2 + ra * NI;
-                    double s = 0.0, pab = 1.0;
-                    for (int k = ib; k >= 0; --k) {
-                        double scd = 0.0, pcd = 1.0;
-                        for (int l = id; l >= 0; --l) {
-                            scd = fma(binom(id, l) * pcd, I[(ia + k) * F1 + ic + l], scd);
-                            pcd *= CDq;
+                    for (int g = lane; g < NG; g += 64) {
+                        const int ia = tb.g_a[g], ib = tb.g_b[g], ic = tb.g_c[g], id = tb.g_d[g];
+                        double s = 0.0, pab = 1.0;
+                        for (int k = ib; k >= 0; --k) {
+                            double scd = 0.0, pcd = 1.0;
+                            for (int l = id; l >= 0; --l) {
+                                scd = fma(binom(id, l) * pcd, I[(ia + k) * F1 + ic + l], scd);
+                                pcd *= CDq;
+                            }
+                            s = fma(binom(ib, k) * pab, scd, s);
+                            pab *= ABq;
                         }
-                        s = fma(binom(ib, k) * pab, scd, s);
-                        pab *= ABq;
+                        G[ra * NG + g] = s;
                     }
-                    G[t] = s;
                 }
                 __syncthreads();
-                for (int i = lane; i < NABCD; i += 64) {
-                    int t = i;
-                    const int id = t % ND;
-                    t /= ND;
-                    const int ic = t % NC;
-                    t /= NC;
-                    const int ib = t % NB;
-                    const int ia = t / NB;
-                    int o[3];
-#pragma unroll
-                    for (int q3 = 0; q3 < 3; ++q3)
-                        o[q3] = ((kCartTab[la][ia][q3] * D2 + kCartTab[lb][ib][q3]) * D3 + kCartTab[lc][ic][q3]) * D4 +
-                                kCartTab[ld][id][q3];
-                    double s = 0.0;
-                    for (int r = 0; r < n; ++r)
-                        s = fma(G[(r * 3) * NG + o[0]] * G[(r * 3 + 1) * NG + o[1]], G[(r * 3 + 2) * NG + o[2]], s);
-                    acc[i] += s;
+                {
+                    int ab = ab0, cd = cd0;
+                    for (int i = lane; i < NABCD; i += 64) {
+                        const int o0 = tb.ab_g[ab][0] + tb.cd_g[cd][0], o1 = tb.ab_g[ab][1] + tb.cd_g[cd][1],
+                                  o2 = tb.ab_g[ab][2] + tb.cd_g[cd][2];
+                        double s = 0.0;
+                        for (int r = 0; r < n; ++r)
+                            s = fma(G[(r * 3) * NG + o0] * G[(r * 3 + 1) * NG + o1], G[(r * 3 + 2) * NG + o2], s);
+                        acc[i] += s;
+                        cd += scd;
+                        ab += sab;
+                        if (cd >= NCD) cd -= NCD, ++ab;
+                    }
                 }
                 __syncthreads();
             }
@@ -263,41 +296,48 @@ __global__ __launch_bounds__(64) void k_generic(GenArgs a) {
         if (bra == ket) deg *= 0.5;
         const bool all_pass = pb.qmin * pk.qmin >= a.tol;
         // apply degeneracy and the function-level Schwarz test in place
-        for (int i = lane; i < NABCD; i += 64) {
-            double m = deg;
-            if (!all_pass) {
-                const int cd = i % NCD, ab = i / NCD;
-                const int af = pb.f0 + ab / NB, bf = pb.f1 + ab % NB, cf = pk.f0 + cd / ND, df = pk.f1 + cd % ND;
-                if (Qm[(size_t)af * nb + bf] * Qm[(size_t)cf * nb + df] < a.tol) m = 0.0;
+        {
+            int ab = ab0, cd = cd0;
+            for (int i = lane; i < NABCD; i += 64) {
+                double m = deg;
+                if (!all_pass) {
+                    const int af = pb.f0 + tb.ab_a[ab], bf = pb.f1 + tb.ab_b[ab];
+                    const int cf = pk.f0 + tb.cd_c[cd], df = pk.f1 + tb.cd_d[cd];
+                    if (Qm[(size_t)af * nb + bf] * Qm[(size_t)cf * nb + df] < a.tol) m = 0.0;
+                }
+                acc[i] *= m;
+                cd += scd;
+                ab += sab;
+                if (cd >= NCD) cd -= NCD, ++ab;
             }
-            acc[i] *= m;
         }
         __syncthreads();
         auto Pv = [&](int r, int c) { return P[(size_t)r * nb + c]; };
+        auto at = [&](int ia, int ib, int ic, int id) { return acc[(ia * NB + ib) * NCD + ic * ND + id]; };
         // A_ab += 2 sum_cd v P_cd ; A_cd += 2 sum_ab v P_ab
-        seg_reduce(lane, NA * NB, NCD, [&](int ab, int cd) {
-            return acc[ab * NCD + cd] * Pv(pk.f0 + cd / ND, pk.f1 + cd % ND);
-        }, [&](int ab, double v) { unsafeAtomicAdd(&Am[(size_t)(pb.f0 + ab / NB) * nb + pb.f1 + ab % NB], 2.0 * v); });
-        seg_reduce(lane, NCD, NA * NB, [&](int cd, int ab) {
-            return acc[ab * NCD + cd] * Pv(pb.f0 + ab / NB, pb.f1 + ab % NB);
-        }, [&](int cd, double v) { unsafeAtomicAdd(&Am[(size_t)(pk.f0 + cd / ND) * nb + pk.f1 + cd % ND], 2.0 * v); });
+        seg_reduce(lane, NAB, NCD, [&](int ab, int cd) {
+            return acc[ab * NCD + cd] * Pv(pk.f0 + tb.cd_c[cd], pk.f1 + tb.cd_d[cd]);
+        }, [&](int ab, double v) { unsafeAtomicAdd(&Am[(size_t)(pb.f0 + tb.ab_a[ab]) * nb + pb.f1 + tb.ab_b[ab]], 2.0 * v); });
+        seg_reduce(lane, NCD, NAB, [&](int cd, int ab) {
+            return acc[ab * NCD + cd] * Pv(pb.f0 + tb.ab_a[ab], pb.f1 + tb.ab_b[ab]);
+        }, [&](int cd, double v) { unsafeAtomicAdd(&Am[(size_t)(pk.f0 + tb.cd_c[cd]) * nb + pk.f1 + tb.cd_d[cd]], 2.0 * v); });
         // B_ac += sum_bd v P_bd ; B_bc += sum_ad v P_ad ; B_ad += sum_bc v P_bc ; B_bd += sum_ac v P_ac
         seg_reduce(lane, NA * NC, NB * ND, [&](int o, int t) {
-            const int ia = o / NC, ic = o % NC, ib = t / ND, id = t % ND;
-            return acc[(ia * NB + ib) * NCD + ic * ND + id] * Pv(pb.f1 + ib, pk.f1 + id);
-        }, [&](int o, double v) { unsafeAtomicAdd(&Bm[(size_t)(pb.f0 + o / NC) * nb + pk.f0 + o % NC], v); });
+            const int ib = tb.bd_b[t], id = tb.bd_d[t];
+            return at(tb.ac_a[o], ib, tb.ac_c[o], id) * Pv(pb.f1 + ib, pk.f1 + id);
+        }, [&](int o, double v) { unsafeAtomicAdd(&Bm[(size_t)(pb.f0 + tb.ac_a[o]) * nb + pk.f0 + tb.ac_c[o]], v); });
         seg_reduce(lane, NB * NC, NA * ND, [&](int o, int t) {
-            const int ib = o / NC, ic = o % NC, ia = t / ND, id = t % ND;
-            return acc[(ia * NB + ib) * NCD + ic * ND + id] * Pv(pb.f0 + ia, pk.f1 + id);
-        }, [&](int o, double v) { unsafeAtomicAdd(&Bm[(size_t)(pb.f1 + o / NC) * nb + pk.f0 + o % NC], v); });
+            const int ia = tb.ad_a[t], id = tb.ad_d[t];
+            return at(ia, tb.bc_b[o], tb.bc_c[o], id) * Pv(pb.f0 + ia, pk.f1 + id);
+        }, [&](int o, double v) { unsafeAtomicAdd(&Bm[(size_t)(pb.f1 + tb.bc_b[o]) * nb + pk.f0 + tb.bc_c[o]], v); });
         seg_reduce(lane, NA * ND, NB * NC, [&](int o, int t) {
-            const int ia = o / ND, id = o % ND, ib = t / NC, ic = t % NC;
-            return acc[(ia * NB + ib) * NCD + ic * ND + id] * Pv(pb.f1 + ib, pk.f0 + ic);
-        }, [&](int o, double v) { unsafeAtomicAdd(&Bm[(size_t)(pb.f0 + o / ND) * nb + pk.f1 + o % ND], v); });
+            const int ib = tb.bc_b[t], ic = tb.bc_c[t];
+            return at(tb.ad_a[o], ib, ic, tb.ad_d[o]) * Pv(pb.f1 + ib, pk.f0 + ic);
+        }, [&](int o, double v) { unsafeAtomicAdd(&Bm[(size_t)(pb.f0 + tb.ad_a[o]) * nb + pk.f1 + tb.ad_d[o]], v); });
         seg_reduce(lane, NB * ND, NA * NC, [&](int o, int t) {
-            const int ib = o / ND, id = o % ND, ia = t / NC, ic = t % NC;
-            return acc[(ia * NB + ib) * NCD + ic * ND + id] * Pv(pb.f0 + ia, pk.f0 + ic);
-        }, [&](int o, double v) { unsafeAtomicAdd(&Bm[(size_t)(pb.f1 + o / ND) * nb + pk.f1 + o % ND], v); });
+            const int ia = tb.ac_a[t], ic = tb.ac_c[t];
+            return at(ia, tb.bd_b[o], ic, tb.bd_d[o]) * Pv(pb.f0 + ia, pk.f0 + ic);
+        }, [&](int o, double v) { unsafeAtomicAdd(&Bm[(size_t)(pb.f1 + tb.bd_b[o]) * nb + pk.f1 + tb.bd_d[o]], v); });
         __syncthreads();
     }
 }
