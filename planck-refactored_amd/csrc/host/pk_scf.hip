@@ -14,11 +14,13 @@
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
 
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <deque>
 #include <map>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "pk_host_internal.h"
@@ -680,22 +682,108 @@ int pk_scf_rhf_batch(const pk_basis *const *bases, int nmol, const pk_scf_option
     std::vector<int64_t> lay_off(nmol);
     size_t ntot = 0;
     double *Pp = nullptr, *Gp = nullptr;
-    std::vector<double> hpart(2 * nparts);
     (void)one;
-    auto gemm = [&](Mol &m, rocblas_operation ta, rocblas_operation tb, int k, const double *A, const double *B, double *Cm) {
-        return rocblas_dgemm(hb, ta, tb, m.n, m.n, k, &one, A, m.n, B, m.n, &zero, Cm, m.n);
+    int nworkers = 0;
+    std::vector<hipStream_t> wst;
+    std::vector<rocblas_handle> whb;
+    auto gemm_h = [&](rocblas_handle h, Mol &m, rocblas_operation ta, rocblas_operation tb, int k, const double *A,
+                      const double *B, double *Cm) {
+        return rocblas_dgemm(h, ta, tb, m.n, m.n, k, &one, A, m.n, B, m.n, &zero, Cm, m.n);
     };
-    auto density = [&](Mol &m, const double *Fm, double *Pt) -> int {
+    auto gemm = [&](Mol &m, rocblas_operation ta, rocblas_operation tb, int k, const double *A, const double *B,
+                    double *Cm) { return gemm_h(hb, m, ta, tb, k, A, B, Cm); };
+    auto density_h = [&](rocblas_handle h, Mol &m, const double *Fm, double *Pt) -> int {
         const int n = m.n;
-        if (gemm(m, rocblas_operation_none, rocblas_operation_none, n, m.X, Fm, m.M) != rocblas_status_success ||
-            gemm(m, rocblas_operation_none, rocblas_operation_none, n, m.M, m.X, m.W1) != rocblas_status_success ||
-            rocsolver_dsyevd(hb, rocblas_evect_original, rocblas_fill_upper, n, m.W1, n, m.ev, m.ee, m.info) !=
+        if (gemm_h(h, m, rocblas_operation_none, rocblas_operation_none, n, m.X, Fm, m.M) != rocblas_status_success ||
+            gemm_h(h, m, rocblas_operation_none, rocblas_operation_none, n, m.M, m.X, m.W1) != rocblas_status_success ||
+            rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, n, m.W1, n, m.ev, m.ee, m.info) !=
                 rocblas_status_success ||
-            gemm(m, rocblas_operation_none, rocblas_operation_none, n, m.X, m.W1, m.C) != rocblas_status_success ||
-            rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_transpose, n, n, m.nocc, &two, m.C, n, m.C, n,
+            gemm_h(h, m, rocblas_operation_none, rocblas_operation_none, n, m.X, m.W1, m.C) != rocblas_status_success ||
+            rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, n, n, m.nocc, &two, m.C, n, m.C, n,
                           &zero, Pt, n) != rocblas_status_success)
             return pkh::fail(PK_ERR_HIP, "pk_scf_rhf_batch: dense algebra failed");
         return PK_OK;
+    };
+    auto density = [&](Mol &m, const double *Fm, double *Pt) -> int { return density_h(hb, m, Fm, Pt); };
+
+    // one SCF iteration of one molecule after the batched build (the body of
+    // pk_scf_rhf's iteration), on stream s2 with handle h
+    auto step = [&](rocblas_handle h, hipStream_t s2, Mol &m, pk_scf_result &r, int it, double build_ms,
+                    std::vector<double> &hpart) -> int {
+#define BSTEP_RB(x)                                                                             \
+    do {                                                                                        \
+        if ((x) != rocblas_status_success) return pkh::fail(PK_ERR_HIP, #x " failed");          \
+    } while (0)
+#define BSTEP_HIP(x)                                                                            \
+    do {                                                                                        \
+        hipError_t e_ = (x);                                                                    \
+        if (e_ != hipSuccess) return pkh::fail(PK_ERR_HIP, std::string(#x ": ") + hipGetErrorString(e_)); \
+    } while (0)
+        const int n = m.n, g2 = (int)((m.n2 + 255) / 256);
+        const size_t n2 = m.n2;
+        r.fock_ms += build_ms;
+        hipLaunchKernelGGL(k_add, dim3(g2), dim3(256), 0, s2, m.H, m.G, m.F, n2);
+        double eph = 0, epf = 0;
+        BSTEP_RB(rocblas_ddot(h, (rocblas_int)n2, m.P, 1, m.H, 1, &eph));
+        BSTEP_RB(rocblas_ddot(h, (rocblas_int)n2, m.P, 1, m.F, 1, &epf));
+        const double Etot = 0.5 * (eph + epf) + r.nuclear_repulsion;
+        BSTEP_RB(gemm_h(h, m, rocblas_operation_none, rocblas_operation_none, n, m.F, m.P, m.W1));
+        BSTEP_RB(gemm_h(h, m, rocblas_operation_none, rocblas_operation_none, n, m.W1, m.S, m.M));
+        hipLaunchKernelGGL(k_antisym, dim3(g2), dim3(256), 0, s2, m.M, m.D, n);
+        BSTEP_RB(gemm_h(h, m, rocblas_operation_none, rocblas_operation_none, n, m.X, m.D, m.W1));
+        int slot;
+        if ((int)m.slots.size() < diis_dim) {
+            slot = (int)m.slots.size();
+        } else {
+            slot = m.slots.front();
+            m.slots.pop_front();
+        }
+        BSTEP_RB(gemm_h(h, m, rocblas_operation_none, rocblas_operation_none, n, m.W1, m.X, m.Eh[slot]));
+        BSTEP_HIP(hipMemcpyAsync(m.Fh[slot], m.F, n2 * 8, hipMemcpyDeviceToDevice, s2));
+        m.slots.push_back(slot);
+        for (int s3 : m.slots) {
+            double v = 0;
+            BSTEP_RB(rocblas_ddot(h, (rocblas_int)n2, m.Eh[slot], 1, m.Eh[s3], 1, &v));
+            m.Bm[slot][s3] = m.Bm[s3][slot] = v;
+        }
+        if (it == 1) r.first_diis_error = std::sqrt(m.Bm[slot][slot] / (double)n2);
+        const double *Fd = m.F;
+        if (m.slots.size() >= 2) {
+            const int k = (int)m.slots.size();
+            std::vector<double> A((size_t)(k + 1) * (k + 1), 0.0), rhs(k + 1, 0.0);
+            for (int a2 = 0; a2 < k; ++a2) {
+                for (int b2 = 0; b2 < k; ++b2) A[(size_t)a2 * (k + 1) + b2] = m.Bm[m.slots[a2]][m.slots[b2]];
+                A[(size_t)a2 * (k + 1) + k] = A[(size_t)k * (k + 1) + a2] = -1.0;
+            }
+            rhs[k] = -1.0;
+            const std::vector<double> cf = qr_solve(k + 1, A, rhs);
+            BSTEP_HIP(hipMemsetAsync(m.M, 0, n2 * 8, s2));
+            for (int a2 = 0; a2 < k; ++a2) BSTEP_RB(rocblas_daxpy(h, (rocblas_int)n2, &cf[a2], m.Fh[m.slots[a2]], 1, m.M, 1));
+            BSTEP_HIP(hipMemcpyAsync(m.D, m.M, n2 * 8, hipMemcpyDeviceToDevice, s2));
+            Fd = m.D;
+        }
+        const int rc3 = density_h(h, m, Fd, m.Pn);
+        if (rc3 != PK_OK) return rc3;
+        hipLaunchKernelGGL(k_diff, dim3(nparts), dim3(256), 0, s2, m.Pn, m.P, n2, m.part);
+        BSTEP_HIP(hipMemcpyAsync(hpart.data(), m.part, 2 * nparts * 8, hipMemcpyDeviceToHost, s2));
+        BSTEP_HIP(hipMemcpyAsync(m.P, m.Pn, n2 * 8, hipMemcpyDeviceToDevice, s2));
+        BSTEP_HIP(hipStreamSynchronize(s2));
+        double dmax = 0, dss = 0;
+        for (int k = 0; k < nparts; ++k) {
+            dmax = std::max(dmax, hpart[2 * k]);
+            dss += hpart[2 * k + 1];
+        }
+        const double drms = std::sqrt(dss / (double)n2), dE = std::fabs(Etot - m.Eprev);
+        m.Eprev = Etot;
+        r.iterations = it;
+        r.energy = Etot;
+        if (it > 1 && dE < opt->tol_energy && drms < opt->tol_density && dmax < opt->tol_density) {
+            r.converged = 1;
+            m.done = true;
+        }
+        return PK_OK;
+#undef BSTEP_RB
+#undef BSTEP_HIP
     };
 
     SCF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
@@ -780,6 +868,18 @@ int pk_scf_rhf_batch(const pk_basis *const *bases, int nmol, const pk_scf_option
             goto done;
         }
     }
+    // workers: the per-molecule algebra of an iteration runs on up to 8 host
+    // threads, each with its own stream and rocBLAS handle (the small dense
+    // calls are launch-bound); the batched build stays on the main stream
+    nworkers = std::max(1, std::min(8, nmol));
+    wst.assign(nworkers, nullptr);
+    whb.assign(nworkers, nullptr);
+    for (int w = 0; w < nworkers; ++w) {
+        SCF_HIP(hipStreamCreateWithFlags(&wst[w], hipStreamNonBlocking));
+        SCF_RB(rocblas_create_handle(&whb[w]));
+        SCF_RB(rocblas_set_stream(whb[w], wst[w]));
+        SCF_RB(rocblas_set_pointer_mode(whb[w], rocblas_pointer_mode_host));
+    }
     for (int it = 1; it <= max_cycles; ++it) {
         SCF_PK(pk_fock_2e(ctx, Pp, Gp, PK_MEM_DEVICE));
         double build_ms = 0;
@@ -788,74 +888,36 @@ int pk_scf_rhf_batch(const pk_basis *const *bases, int nmol, const pk_scf_option
             pk_ctx_stats(ctx, &s);
             build_ms = s.last_build_ms;
         }
-        bool all = true;
-        for (int i = 0; i < nmol; ++i) {
-            Mol &m = mols[i];
-            if (m.done) continue;
-            const int n = m.n, g2 = (int)((m.n2 + 255) / 256);
-            const size_t n2 = m.n2;
-            res[i].fock_ms += build_ms;
-            hipLaunchKernelGGL(k_add, dim3(g2), dim3(256), 0, st, m.H, m.G, m.F, n2);
-            double eph = 0, epf = 0;
-            SCF_RB(rocblas_set_pointer_mode(hb, rocblas_pointer_mode_host));
-            SCF_RB(rocblas_ddot(hb, (rocblas_int)n2, m.P, 1, m.H, 1, &eph));
-            SCF_RB(rocblas_ddot(hb, (rocblas_int)n2, m.P, 1, m.F, 1, &epf));
-            const double Etot = 0.5 * (eph + epf) + res[i].nuclear_repulsion;
-            SCF_RB(gemm(m, rocblas_operation_none, rocblas_operation_none, n, m.F, m.P, m.W1));
-            SCF_RB(gemm(m, rocblas_operation_none, rocblas_operation_none, n, m.W1, m.S, m.M));
-            hipLaunchKernelGGL(k_antisym, dim3(g2), dim3(256), 0, st, m.M, m.D, n);
-            SCF_RB(gemm(m, rocblas_operation_none, rocblas_operation_none, n, m.X, m.D, m.W1));
-            int slot;
-            if ((int)m.slots.size() < diis_dim) {
-                slot = (int)m.slots.size();
-            } else {
-                slot = m.slots.front();
-                m.slots.pop_front();
-            }
-            SCF_RB(gemm(m, rocblas_operation_none, rocblas_operation_none, n, m.W1, m.X, m.Eh[slot]));
-            SCF_HIP(hipMemcpyAsync(m.Fh[slot], m.F, n2 * 8, hipMemcpyDeviceToDevice, st));
-            m.slots.push_back(slot);
-            for (int s2 : m.slots) {
-                double v = 0;
-                SCF_RB(rocblas_ddot(hb, (rocblas_int)n2, m.Eh[slot], 1, m.Eh[s2], 1, &v));
-                m.Bm[slot][s2] = m.Bm[s2][slot] = v;
-            }
-            if (it == 1) res[i].first_diis_error = std::sqrt(m.Bm[slot][slot] / (double)n2);
-            const double *Fd = m.F;
-            if (m.slots.size() >= 2) {
-                const int k = (int)m.slots.size();
-                std::vector<double> A((size_t)(k + 1) * (k + 1), 0.0), rhs(k + 1, 0.0);
-                for (int a = 0; a < k; ++a) {
-                    for (int b2 = 0; b2 < k; ++b2) A[(size_t)a * (k + 1) + b2] = m.Bm[m.slots[a]][m.slots[b2]];
-                    A[(size_t)a * (k + 1) + k] = A[(size_t)k * (k + 1) + a] = -1.0;
+        SCF_HIP(hipStreamSynchronize(st));
+        std::atomic<int> next{0};
+        std::vector<int> wrc(nworkers, PK_OK);
+        std::vector<std::string> werr(nworkers);
+        auto worker = [&](int w) {
+            rocblas_handle h = whb[w];
+            hipStream_t s2 = wst[w];
+            std::vector<double> hpart(2 * nparts);
+            for (int i; (i = next.fetch_add(1)) < nmol;) {
+                Mol &m = mols[i];
+                if (m.done) continue;
+                const int rc2 = step(h, s2, m, res[i], it, build_ms, hpart);
+                if (rc2 != PK_OK) {
+                    wrc[w] = rc2;
+                    werr[w] = pkh::g_err;
+                    return;
                 }
-                rhs[k] = -1.0;
-                const std::vector<double> cf = qr_solve(k + 1, A, rhs);
-                SCF_HIP(hipMemsetAsync(m.M, 0, n2 * 8, st));
-                for (int a = 0; a < k; ++a) SCF_RB(rocblas_daxpy(hb, (rocblas_int)n2, &cf[a], m.Fh[m.slots[a]], 1, m.M, 1));
-                SCF_HIP(hipMemcpyAsync(m.D, m.M, n2 * 8, hipMemcpyDeviceToDevice, st));
-                Fd = m.D;
             }
-            if ((rc = density(m, Fd, m.Pn)) != PK_OK) goto done;
-            hipLaunchKernelGGL(k_diff, dim3(nparts), dim3(256), 0, st, m.Pn, m.P, n2, m.part);
-            SCF_HIP(hipMemcpyAsync(hpart.data(), m.part, 2 * nparts * 8, hipMemcpyDeviceToHost, st));
-            SCF_HIP(hipMemcpyAsync(m.P, m.Pn, n2 * 8, hipMemcpyDeviceToDevice, st));
-            SCF_HIP(hipStreamSynchronize(st));
-            double dmax = 0, dss = 0;
-            for (int k = 0; k < nparts; ++k) {
-                dmax = std::max(dmax, hpart[2 * k]);
-                dss += hpart[2 * k + 1];
+        };
+        std::vector<std::thread> th;
+        for (int w = 1; w < nworkers; ++w) th.emplace_back(worker, w);
+        worker(0);
+        for (std::thread &t : th) t.join();
+        for (int w = 0; w < nworkers; ++w)
+            if (wrc[w] != PK_OK) {
+                rc = pkh::fail(wrc[w], werr[w]);
+                goto done;
             }
-            const double drms = std::sqrt(dss / (double)n2), dE = std::fabs(Etot - m.Eprev);
-            m.Eprev = Etot;
-            res[i].iterations = it;
-            res[i].energy = Etot;
-            if (it > 1 && dE < opt->tol_energy && drms < opt->tol_density && dmax < opt->tol_density) {
-                res[i].converged = 1;
-                m.done = true;
-            }
-            all = all && m.done;
-        }
+        bool all = true;
+        for (const Mol &m : mols) all = all && m.done;
         if (all) break;
     }
     for (int i = 0; i < nmol; ++i)
@@ -869,6 +931,10 @@ done:
     {
         const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
         for (int i = 0; i < nmol; ++i) res[i].wall_s = wall;
+    }
+    for (size_t w = 0; w < whb.size(); ++w) {
+        if (whb[w]) rocblas_destroy_handle(whb[w]);
+        if (wst[w]) (void)hipStreamDestroy(wst[w]);
     }
     if (ctx) pk_ctx_destroy(ctx);
     if (hb) rocblas_destroy_handle(hb);
