@@ -36,7 +36,8 @@ int pk_basis_nbasis(const pk_basis *b);
 /* pointers into b, valid until pk_basis_free */
 int pk_basis_describe(const pk_basis *b, pk_basis_desc *desc);
 
-/* S, T, V (nb x nb, host memory; any may be NULL). */
+/* S, T, V (nb x nb, host memory; any may be NULL), computed on the device by
+ * a HIP kernel; PK_ERR_HIP when no HIP device is visible (no CPU path). */
 int pk_one_electron(const pk_basis *b, double *S, double *T, double *V);
 double pk_nuclear_repulsion(const pk_basis *b);
 

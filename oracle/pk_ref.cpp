@@ -199,10 +199,30 @@ void init_boys_table() {
     }
 }
 
+// Not the reference: F_n(x) to ~1e-17 relative, for checks that must see past
+// the lookup's ~1e-12 error.  Below 157.1 the series
+//   F_n(x) = e^{-x} sum_k (2x)^k / ((2n+1)(2n+3)...(2n+2k+1))
+// in long double (x87, 64-bit mantissa; all terms positive, so the sum is
+// well conditioned); beyond, the asymptotic form (its neglected part is below
+// e^{-157}).  Selected by pkref_set_boys_exact(1).
+int g_boys_exact = 0;
+double boys_exact(int n, double x) {
+    if (x >= kBoysRows * kBoysStep) return std::tgamma(n + 0.5) / (2.0 * std::pow(x, n + 0.5));
+    const long double X = x;
+    long double term = 1.0L / (2 * n + 1), sum = term;
+    for (int k = 1; k < 4000; ++k) {
+        term *= 2 * X / (2 * n + 2 * k + 1);
+        sum += term;
+        if (term < sum * 1e-22L) break;
+    }
+    return (double)(expl(-X) * sum);
+}
+
 // boys.cpp:1592-1618: asymptotic form at and beyond 157.1, else a 6-term Taylor
 // expansion from the grid point below x.
 double boys(int n, double x) {
     if (n < 0 || n >= kBoysCols) return std::nan("");
+    if (g_boys_exact) return boys_exact(n, x);
     const double x_max = kBoysRows * kBoysStep;
     if (x >= x_max) return std::tgamma(n + 0.5) / (2.0 * std::pow(x, n + 0.5));
     static const double fact[7] = {1.0, 1.0, 2.0, 6.0, 24.0, 120.0, 720.0};
@@ -1291,6 +1311,8 @@ double pkref_boys(int n, double x) {
     init_boys_table();
     return boys(n, x);
 }
+
+void pkref_set_boys_exact(int on) { g_boys_exact = on ? 1 : 0; }
 
 void pkref_rys_rule(int n, double T, double *roots, double *weights) { rys_rule(n, T, roots, weights); }
 

@@ -56,6 +56,10 @@ int pkref_rhf(pkref_ctx *ctx, int charge, double tol_energy, double tol_density,
 
 /* Boys F_n(x) by the table + 6-term Taylor lookup */
 double pkref_boys(int n, double x);
+/* 1: every later Boys evaluation (pkref_boys, the ERIs, the nuclear
+ * attraction) uses the accurate series instead of the reference's lookup;
+ * 0: the lookup (default).  Process-wide; not the reference's behaviour. */
+void pkref_set_boys_exact(int on);
 /* n-point Rys rule by the moment/Stieltjes/Jacobi construction */
 void pkref_rys_rule(int n, double T, double *roots, double *weights);
 /* auto-engine routing decision for one Cartesian component quartet */

@@ -6,12 +6,8 @@
 //   per-atom shells in file order, Nc scaling ...... src/basis/gaussian.cpp:183-302
 //   primitive / contracted normalisation ........... src/basis/basis.cpp:39-75
 //   Cartesian component order and norms ............ src/basis/basis.cpp:20-37, gaussian.cpp:286-298
-//   overlap / kinetic ............................... src/integrals/os.cpp:248-395
-//   nuclear attraction .............................. src/integrals/os.cpp:462-602
 //   nuclear repulsion ............................... src/base/types.h:934-964
-// The one-electron integrals are built per shell pair with all Cartesian
-// components at once: 1-D overlap recurrences for S and T, and a Rys rule
-// (the engine's own table) for the nuclear attraction.
+// The one-electron integrals are computed on the device (pk_one.hip).
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -23,7 +19,6 @@
 #include <vector>
 
 #include "pk_host_internal.h"
-#include "rys_table.inc"
 
 namespace pkh {
 
@@ -124,32 +119,6 @@ int dfac(int n) {
     return r;
 }
 
-// host evaluation of the engine's Rys rule (same table as the kernels)
-void rys_rule_host(int n, double T, double *u, double *w) {
-    if (T >= pk_rys_tmax[n]) {
-        const double it = 1.0 / T, sit = std::sqrt(it);
-        for (int r = 0; r < n; ++r) {
-            u[r] = pk_rys_ax2[pk_rys_abase[n] + r] * it;
-            w[r] = pk_rys_aw[pk_rys_abase[n] + r] * sit;
-        }
-        return;
-    }
-    const int i = (int)T;
-    const double x = 2.0 * (T - i) - 1.0, x2 = 2.0 * x;
-    const double *c = pk_rys_coef + pk_rys_base[n] + i * (2 * n * (PK_RYS_DEG + 1));
-    for (int f = 0; f < 2 * n; ++f) {
-        const double *cf = c + f * (PK_RYS_DEG + 1);
-        double b1 = 0, b2 = 0;
-        for (int j = PK_RYS_DEG; j >= 1; --j) {
-            const double t = std::fma(x2, b1, cf[j] - b2);
-            b2 = b1;
-            b1 = t;
-        }
-        const double v = std::fma(x, b1, cf[0] - b2);
-        (f < n ? u[f] : w[f - n]) = v;
-    }
-}
-
 int ncart(int l) { return (l + 1) * (l + 2) / 2; }
 void cart(int l, int k, int &x, int &y, int &z) {
     int lx = l;
@@ -214,134 +183,6 @@ double nuclear_repulsion(const Basis &b) {
             E += (double)b.Z[i] * b.Z[j] / std::sqrt(dx * dx + dy * dy + dz * dz);
         }
     return E;
-}
-
-// S, T, V over shell pairs (row-major nb x nb, symmetric)
-int one_electron(const Basis &b, double *S, double *T, double *V) {
-    const int ns = (int)b.l.size(), nb = b.nb;
-    std::vector<int> f0(ns), p0(ns);
-    for (int s = 0, f = 0, p = 0; s < ns; ++s) {
-        f0[s] = f;
-        p0[s] = p;
-        f += ncart(b.l[s]);
-        p += b.nprim[s];
-    }
-    const int nat = (int)b.Z.size();
-#pragma omp parallel for schedule(dynamic)
-    for (int A = 0; A < ns; ++A) {
-        for (int Bs = 0; Bs <= A; ++Bs) {
-            const int la = b.l[A], lb = b.l[Bs], na = ncart(la), nbb = ncart(lb);
-            const double *Ac = &b.center[3 * A], *Bc = &b.center[3 * Bs];
-            double AB[3], R2 = 0;
-            for (int q = 0; q < 3; ++q) {
-                AB[q] = Ac[q] - Bc[q];
-                R2 += AB[q] * AB[q];
-            }
-            std::vector<double> s(na * nbb, 0.0), t(na * nbb, 0.0), v(na * nbb, 0.0);
-            for (int i = 0; i < b.nprim[A]; ++i)
-                for (int j = 0; j < b.nprim[Bs]; ++j) {
-                    const double al = b.exponent[p0[A] + i], be = b.exponent[p0[Bs] + j], ze = al + be;
-                    const double cc = b.coef[p0[A] + i] * b.prim_norm[p0[A] + i] * b.coef[p0[Bs] + j] * b.prim_norm[p0[Bs] + j];
-                    const double E = std::exp(-al * be / ze * R2) * cc;
-                    double P[3], PA[3], PB[3];
-                    for (int q = 0; q < 3; ++q) {
-                        P[q] = (al * Ac[q] + be * Bc[q]) / ze;
-                        PA[q] = P[q] - Ac[q];
-                        PB[q] = P[q] - Bc[q];
-                    }
-                    // 1-D overlaps O[q][i][j], i <= la, j <= lb + 2 (unit seed)
-                    double O[3][4][6];
-                    const double h = 0.5 / ze;
-                    for (int q = 0; q < 3; ++q) {
-                        for (int ii = 0; ii <= la; ++ii)
-                            for (int jj = 0; jj <= lb + 2; ++jj) {
-                                double val;
-                                if (ii == 0 && jj == 0)
-                                    val = 1.0;
-                                else if (ii > 0) {
-                                    val = PA[q] * O[q][ii - 1][jj];
-                                    if (ii > 1) val += (ii - 1) * h * O[q][ii - 2][jj];
-                                    if (jj > 0) val += jj * h * O[q][ii - 1][jj - 1];
-                                } else {
-                                    val = PB[q] * O[q][0][jj - 1];
-                                    if (jj > 1) val += (jj - 1) * h * O[q][0][jj - 2];
-                                }
-                                O[q][ii][jj] = val;
-                            }
-                    }
-                    const double sf = std::pow(M_PI / ze, 1.5) * E;
-                    for (int ka = 0; ka < na; ++ka) {
-                        int a[3];
-                        cart(la, ka, a[0], a[1], a[2]);
-                        for (int kb = 0; kb < nbb; ++kb) {
-                            int bq[3];
-                            cart(lb, kb, bq[0], bq[1], bq[2]);
-                            double o[3], k1[3];
-                            for (int q = 0; q < 3; ++q) {
-                                const int x = a[q], y = bq[q];
-                                o[q] = O[q][x][y];
-                                k1[q] = be * (2 * y + 1) * O[q][x][y] - 2.0 * be * be * O[q][x][y + 2];
-                                if (y >= 2) k1[q] -= 0.5 * y * (y - 1) * O[q][x][y - 2];
-                            }
-                            s[ka * nbb + kb] += sf * o[0] * o[1] * o[2];
-                            t[ka * nbb + kb] += sf * (k1[0] * o[1] * o[2] + o[0] * k1[1] * o[2] + o[0] * o[1] * k1[2]);
-                        }
-                    }
-                    // nuclear attraction, Rys rule with n = (la+lb)/2 + 1 roots
-                    const int n = (la + lb) / 2 + 1;
-                    const double vf = 2.0 * M_PI / ze * E;
-                    for (int c = 0; c < nat; ++c) {
-                        const double *C = &b.xyz[3 * c];
-                        double PC[3], T2 = 0;
-                        for (int q = 0; q < 3; ++q) {
-                            PC[q] = P[q] - C[q];
-                            T2 += PC[q] * PC[q];
-                        }
-                        double u[8], w[8];
-                        rys_rule_host(n, ze * T2, u, w);
-                        for (int r = 0; r < n; ++r) {
-                            // 1-D integrals I[q][i][j], i <= la, j <= lb by VRR on i then HRR to j
-                            double I[3][4][4];
-                            for (int q = 0; q < 3; ++q) {
-                                double g[8];
-                                const double c00 = PA[q] - u[r] * PC[q], b10 = (1.0 - u[r]) * h;
-                                g[0] = 1.0;
-                                for (int e = 1; e <= la + lb; ++e) {
-                                    g[e] = c00 * g[e - 1];
-                                    if (e > 1) g[e] += (e - 1) * b10 * g[e - 2];
-                                }
-                                double H2[8][4];
-                                for (int e = 0; e <= la + lb; ++e) H2[e][0] = g[e];
-                                for (int jj = 1; jj <= lb; ++jj)
-                                    for (int e = 0; e <= la + lb - jj; ++e) H2[e][jj] = H2[e + 1][jj - 1] + AB[q] * H2[e][jj - 1];
-                                for (int ii = 0; ii <= la; ++ii)
-                                    for (int jj = 0; jj <= lb; ++jj) I[q][ii][jj] = H2[ii][jj];
-                            }
-                            const double fw = -(double)b.Z[c] * vf * w[r];
-                            for (int ka = 0; ka < na; ++ka) {
-                                int a[3];
-                                cart(la, ka, a[0], a[1], a[2]);
-                                for (int kb = 0; kb < nbb; ++kb) {
-                                    int bq[3];
-                                    cart(lb, kb, bq[0], bq[1], bq[2]);
-                                    v[ka * nbb + kb] += fw * I[0][a[0]][bq[0]] * I[1][a[1]][bq[1]] * I[2][a[2]][bq[2]];
-                                }
-                            }
-                        }
-                    }
-                }
-            for (int ka = 0; ka < na; ++ka)
-                for (int kb = 0; kb < nbb; ++kb) {
-                    const int i = f0[A] + ka, j = f0[Bs] + kb;
-                    const double nn = b.fnorm[i] * b.fnorm[j];
-                    const size_t x = (size_t)i * nb + j, y = (size_t)j * nb + i;
-                    if (S) S[x] = S[y] = nn * s[ka * nbb + kb];
-                    if (T) T[x] = T[y] = nn * t[ka * nbb + kb];
-                    if (V) V[x] = V[y] = nn * v[ka * nbb + kb];
-                }
-        }
-    }
-    return PK_OK;
 }
 
 }  // namespace pkh

@@ -7,6 +7,8 @@
 
 #include "pk_scf.h"
 
+typedef struct ihipStream_t *hipStream_t;
+
 namespace pkh {
 
 struct Basis {
@@ -23,7 +25,11 @@ extern thread_local std::string g_err;
 int fail(int code, const std::string &m);
 int load_basis(const char *path, int natoms, const int32_t *Z, const double *xyz, Basis &b);
 double nuclear_repulsion(const Basis &b);
+// S, T, V (row-major nb x nb, symmetric) computed on the device: host
+// pointers (one_electron) or device pointers on the caller's stream
+// (one_electron_device, returns after the kernel completed); null = skip
 int one_electron(const Basis &b, double *S, double *T, double *V);
+int one_electron_device(const Basis &b, double *dS, double *dT, double *dV, hipStream_t st);
 // the shells of atom a moved to the origin (a one-atom basis of the same element)
 Basis atom_basis(const Basis &b, int a);
 // symmetric eigenproblem by cyclic Jacobi rotations (small matrices):

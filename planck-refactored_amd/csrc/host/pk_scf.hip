@@ -421,7 +421,7 @@ int pk_scf_rhf(const pk_basis *bas, const pk_scf_options *opt, pk_scf_result *re
     pk_ctx *ctx = nullptr;
     rocblas_handle hb = nullptr;
     hipStream_t st = nullptr;
-    std::vector<double> hS(n2), hT(n2), hV(n2), hH(n2), w(n);
+    std::vector<double> w(n);
     double *S, *H, *X, *P, *Pn, *F, *G, *M, *D, *W1, *C, *ev, *ee, *part;
     rocblas_int *info;
     std::deque<int> slots;  // DIIS ring: slot indices, oldest first
@@ -455,9 +455,6 @@ int pk_scf_rhf(const pk_basis *bas, const pk_scf_options *opt, pk_scf_result *re
         return PK_OK;
     };
 
-    pkh::one_electron(b, hS.data(), hT.data(), hV.data());
-    for (size_t i = 0; i < n2; ++i) hH[i] = hT[i] + hV[i];
-
     SCF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     SCF_RB(rocblas_create_handle(&hb));
     SCF_RB(rocblas_set_stream(hb, st));
@@ -480,8 +477,9 @@ int pk_scf_rhf(const pk_basis *bas, const pk_scf_options *opt, pk_scf_result *re
         goto done;
     }
     Bm.assign(diis_dim, std::vector<double>(diis_dim, 0.0));
-    SCF_HIP(hipMemcpyAsync(S, hS.data(), n2 * 8, hipMemcpyHostToDevice, st));
-    SCF_HIP(hipMemcpyAsync(H, hH.data(), n2 * 8, hipMemcpyHostToDevice, st));
+    // S, and H = T + V, on the device (T through M, V through G: scratch until the first build)
+    if ((rc = pkh::one_electron_device(b, S, M, G, st)) != PK_OK) goto done;
+    hipLaunchKernelGGL(k_add, dim3(g2), dim3(256), 0, st, M, G, H, n2);
     // X = U s^{-1/2} U^T
     SCF_HIP(hipMemcpyAsync(W1, S, n2 * 8, hipMemcpyDeviceToDevice, st));
     SCF_RB(rocsolver_dsyevd(hb, rocblas_evect_original, rocblas_fill_upper, n, W1, n, ev, ee, info));
@@ -844,11 +842,9 @@ int pk_scf_rhf_batch(const pk_basis *const *bases, int nmol, const pk_scf_option
         m.P = Pp + lay_off[i];
         m.G = Gp + lay_off[i];
         {
-            std::vector<double> hS(n2), hT(n2), hV(n2), w(n);
-            pkh::one_electron(*m.b, hS.data(), hT.data(), hV.data());
-            for (size_t k = 0; k < n2; ++k) hT[k] += hV[k];
-            SCF_HIP(hipMemcpyAsync(m.S, hS.data(), n2 * 8, hipMemcpyHostToDevice, st));
-            SCF_HIP(hipMemcpyAsync(m.H, hT.data(), n2 * 8, hipMemcpyHostToDevice, st));
+            std::vector<double> w(n);
+            if ((rc = pkh::one_electron_device(*m.b, m.S, m.M, m.F, st)) != PK_OK) goto done;
+            hipLaunchKernelGGL(k_add, dim3(g2), dim3(256), 0, st, m.M, m.F, m.H, n2);
             SCF_HIP(hipMemcpyAsync(m.W1, m.S, n2 * 8, hipMemcpyDeviceToDevice, st));
             SCF_RB(rocsolver_dsyevd(hb, rocblas_evect_original, rocblas_fill_upper, n, m.W1, n, m.ev, m.ee, m.info));
             SCF_HIP(hipMemcpyAsync(w.data(), m.ev, n * 8, hipMemcpyDeviceToHost, st));

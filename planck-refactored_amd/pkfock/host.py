@@ -102,9 +102,12 @@ class Molecule:
                            np.ctypeslib.as_array(d.prim_norm, shape=(tot,)).copy())
 
     def one_electron(self):
+        """S, T, V (computed on the device; raises PkError without one)."""
         n = self.nbasis
         S, T, V = np.zeros((n, n)), np.zeros((n, n)), np.zeros((n, n))
-        load().pk_one_electron(self.handle, S.ctypes.data, T.ctypes.data, V.ctypes.data)
+        lib = load()
+        if lib.pk_one_electron(self.handle, S.ctypes.data, T.ctypes.data, V.ctypes.data) != 0:
+            raise _eng.PkError("pk_one_electron: " + lib.pk_host_last_error().decode())
         return S, T, V
 
     def nuclear_repulsion(self):

@@ -44,6 +44,7 @@ def lib():
         L.pkref_rhf.argtypes = [vp, ip, d, d, ip, ip, vp, vp, vp, vp, ip]
         L.pkref_boys.restype = d
         L.pkref_boys.argtypes = [ip, d]
+        L.pkref_set_boys_exact.argtypes = [ip]
         L.pkref_rys_rule.argtypes = [ip, d, vp, vp]
         L.pkref_auto_prefers_rys.argtypes = [vp, vp, vp, vp]
         L.pkref_syev.argtypes = [ip, vp, vp, vp]
@@ -143,6 +144,20 @@ class Ref:
         rc = lib().pkref_rhf(self.ctx, charge, tol_energy, tol_density, diis_dim, max_cycles, ctypes.byref(E),
                              ctypes.byref(it), _p(P), _p(hist), len(hist))
         return {"rc": rc, "energy": E.value, "iterations": it.value, "P": P, "history": hist[:it.value]}
+
+
+class exact_boys:
+    """Context manager: the oracle's Boys function switches from the
+    reference's table lookup (~1e-12 absolute) to an accurate series
+    (~1e-17 relative) inside the block."""
+
+    def __enter__(self):
+        lib().pkref_set_boys_exact(1)
+        return self
+
+    def __exit__(self, *exc):
+        lib().pkref_set_boys_exact(0)
+        return False
 
 
 def boys(n, x):

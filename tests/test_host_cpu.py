@@ -1,8 +1,10 @@
-"""Host layer (lib/libpkscf.so) on the CPU: the product's own basis loader and
-one-electron integrals against the oracle, and the C ABI's exported symbols.
+"""Host layer (lib/libpkscf.so) on the CPU: the product's own basis loader
+against the oracle, the C ABI's exported symbols, and the device-only entry
+points refusing to run without a device.
 
-Bars: basis data and S, T to 1e-13 (same formulas up to rounding); V to 1e-10
-(the oracle's Boys lookup is the reference's table + 6-term Taylor, ~1e-12).
+Bars: basis data to 1e-13 (same formulas up to rounding); nuclear repulsion
+to 1e-12.  The one-electron integrals run on the device only
+(tests/test_gpu_one_electron.py).
 """
 import ctypes
 import os
@@ -24,7 +26,7 @@ CASES = [("sto-3g", water()), ("6-31g*", water()), ("cc-pvdz", alkane(2)), ("cc-
 
 
 @pytest.mark.parametrize("basis,mol", CASES, ids=[c[0] + "-%d" % len(c[1][0]) for c in CASES])
-def test_basis_and_one_electron_match_oracle(basis, mol):
+def test_basis_matches_oracle(basis, mol):
     Z, X = mol
     m = host.Molecule(Z, X, basis)
     r = refcpu.Ref(basis, Z, X * ANGSTROM_TO_BOHR)
@@ -36,11 +38,20 @@ def test_basis_and_one_electron_match_oracle(basis, mol):
     assert np.abs(sh.exponent - re_).max() == 0.0
     assert np.abs(sh.coef - rco).max() <= 1e-13 * np.abs(rco).max()
     assert np.abs(sh.prim_norm - rno).max() <= 1e-13 * np.abs(rno).max()
-    S, T, V = m.one_electron()
-    assert np.abs(S - r.matrix("overlap")).max() < 1e-13
-    assert np.abs(T - r.matrix("kinetic")).max() < 1e-12
-    assert np.abs(V - r.matrix("nuclear")).max() < 1e-10
     assert abs(m.nuclear_repulsion() - r.enuc()) < 1e-12
+
+
+def test_one_electron_refuses_without_device():
+    """The one-electron integrals run on the device only (no CPU fallback)."""
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("a HIP device is present")
+    except ImportError:
+        pass
+    m = host.Molecule(*water(), "sto-3g")
+    with pytest.raises(eng_mod.PkError, match="no HIP device"):
+        m.one_electron()
 
 
 def test_basis_errors_are_reported():

@@ -58,6 +58,20 @@ def test_boys_against_mpmath():
     assert worst < 1e-12, worst  # table + 6-term Taylor accuracy of the reference lookup
 
 
+def test_exact_boys_mode_against_mpmath():
+    """The oracle's accurate Boys mode (used to pin device integrals past the
+    lookup's error) against 40-digit mpmath, relative 1e-15."""
+    rng = np.random.default_rng(6)
+    worst = 0.0
+    with refcpu.exact_boys():
+        for x in list(rng.uniform(0, 160, 80)) + [0.0, 1e-9, 0.05, 30.0, 157.05, 157.2, 400.0]:
+            for n in (0, 1, 2, 4, 7, 12, 20):
+                ex = float(F_exact(n, x))
+                worst = max(worst, abs(refcpu.boys(n, float(x)) - ex) / ex)
+    assert refcpu.boys(0, 0.5) != 0.0
+    assert worst < 1e-15, worst
+
+
 def _custom_basis(shells):
     """Temporary Gaussian94 file: element -> list of (label, exponent)."""
     f = tempfile.NamedTemporaryFile("w", suffix=".gbs", delete=False)
