@@ -34,7 +34,7 @@ struct PairDev {
     int f0, f1;
     int prim_off, nprim;
     int diag;  // 1 when both shells are the same shell
-    int pad;
+    int grp;   // ordering group (primitive-count bucket) of the pair as a ket
     double ab[3];
     double qmin;  // smallest Schwarz bound Q_ij over the pair's functions
 };

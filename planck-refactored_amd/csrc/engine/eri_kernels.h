@@ -147,11 +147,12 @@ __global__ __launch_bounds__(256) void k_jk(JKArgs a) {
     sfor<0, NA>([&](auto i_) { kac[PK_CI(i_)] = 0.0; sfor<0, ND>([&](auto d_) { kad[PK_CI(i_)][PK_CI(d_)] = 0.0; }); });
     sfor<0, NB>([&](auto i_) { kbc[PK_CI(i_)] = 0.0; sfor<0, ND>([&](auto d_) { kbd[PK_CI(i_)][PK_CI(d_)] = 0.0; }); });
 
-    // The work lists order each bra's kets by primitive count, then first
-    // shell, so lanes with equal (primitive count, first ket shell) are
-    // contiguous; (a|c) and (b|c) exchange sums are added over such a segment
-    // (segmented scan over the wave) and stored once, by its last lane.
-    const int ckey = active ? (pk.nprim << 20) + pk.f0 : -1 - lane;
+    // The work lists order each bra's kets by ordering group (a bucket of
+    // primitive counts), then first shell, then second shell, so lanes with
+    // equal (group, first ket shell) are contiguous; (a|c) and (b|c) exchange
+    // sums are added over such a segment (segmented scan over the wave) and
+    // stored once, by its last lane.
+    const int ckey = active ? (pk.grp << 20) + pk.f0 : -1 - lane;
     bool seg_same[6];
     sfor<0, 6>([&](auto k_) {
         constexpr int off = 1 << PK_CI(k_);

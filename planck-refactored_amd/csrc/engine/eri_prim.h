@@ -64,15 +64,20 @@ __device__ __forceinline__ PrimQ<R> prim_quartet(const PrimDev &p, const PrimDev
 
 // ---------------------------------------------------------------- Boys
 // F_0..F_M(T): 8-term Taylor series of F_M about the nearest grid point
-// (step 1/16) and downward recursion below 36, with e^{-T} = e^{-x_i} e^{-d};
-// F_0 = sqrt(pi/T)/2 and the upward recursion at and above 36.
+// (step 1/16) and downward recursion below 50, with e^{-T} = e^{-x_i} e^{-d};
+// F_0 = sqrt(pi/T)/2 and the upward recursion at and above 50.  The upward
+// recursion's e^{-T} term is dropped for M <= 4 (the orders of the
+// Obara-Saika kernels): there it is below 2e-17 of F_M, and the large-T
+// branch then costs a reciprocal square root and M products instead of an
+// exponential.
 template <int M>
 __device__ __forceinline__ void boys(double T, double (&F)[M + 1]) {
     static_assert(M + 8 <= PK_BOYS_COLS, "Boys table too narrow for this class");
     if (T < (double)PK_BOYS_XMAX) {
         const int i = (int)fma(T, (double)PK_BOYS_STEP_INV, 0.5);
         const double d = fma((double)i, -1.0 / PK_BOYS_STEP_INV, T);
-        const double *row = pk_boys_table + i * PK_BOYS_COLS + M;
+        // 24-bit multiply (full rate; i < 2^24)
+        const double *row = pk_boys_table + __umul24((unsigned)i, PK_BOYS_COLS) + M;
         double s = row[7];
         s = fma(s, -d * (1.0 / 7.0), row[6]);
         s = fma(s, -d * (1.0 / 6.0), row[5]);
@@ -103,24 +108,33 @@ __device__ __forceinline__ void boys(double T, double (&F)[M + 1]) {
         const double r = rsqrt_nr(T);
         F[0] = 0.88622692545275801365 * r;  // sqrt(pi/T)/2
         if constexpr (M > 0) {
-            const double eT = exp(-T), it2 = 0.5 * r * r;
-            sfor<1, M + 1>([&](auto m_) {
-                constexpr int m = PK_CI(m_);
-                F[m] = ((2 * m - 1) * F[m - 1] - eT) * it2;
-            });
+            const double it2 = 0.5 * r * r;
+            if constexpr (M <= 4) {
+                sfor<1, M + 1>([&](auto m_) {
+                    constexpr int m = PK_CI(m_);
+                    F[m] = (2 * m - 1) * F[m - 1] * it2;
+                });
+            } else {
+                const double eT = exp(-T);
+                sfor<1, M + 1>([&](auto m_) {
+                    constexpr int m = PK_CI(m_);
+                    F[m] = ((2 * m - 1) * F[m - 1] - eT) * it2;
+                });
+            }
         }
     }
 }
 
-// FP32 path: 5-term series from the float tables below 36 (|d| <= 1/32, next
-// term below 3e-10 relative); the large-T branch in FP64.
+// FP32 path: 5-term series from the float tables below 50 (|d| <= 1/32, next
+// term below 3e-10 relative); the large-T branch in FP64 without the e^{-T}
+// term (below 2e-17 of F_M for M <= 4, the orders of the FP32 kernels).
 template <int M>
 __device__ __forceinline__ void boys(float T, float (&F)[M + 1]) {
     static_assert(M + 5 <= PK_BOYS_COLS, "Boys table too narrow for this class");
     if (T < (float)PK_BOYS_XMAX) {
         const int i = (int)fmaf(T, (float)PK_BOYS_STEP_INV, 0.5f);
         const float d = fmaf((float)i, -1.0f / PK_BOYS_STEP_INV, T);
-        const float *row = pk_boys_table_f + i * PK_BOYS_COLS + M;
+        const float *row = pk_boys_table_f + __umul24((unsigned)i, PK_BOYS_COLS) + M;
         float s = row[4];
         s = fmaf(s, -d * (1.0f / 4.0f), row[3]);
         s = fmaf(s, -d * (1.0f / 3.0f), row[2]);
@@ -146,10 +160,11 @@ __device__ __forceinline__ void boys(float T, float (&F)[M + 1]) {
         double f = 0.88622692545275801365 * r;
         F[0] = (float)f;
         if constexpr (M > 0) {
-            const double eT = (double)__expf(-T), it2 = 0.5 * r * r;
+            static_assert(M <= 4, "FP32 large-T Boys branch drops e^{-T}: orders up to 4 only");
+            const double it2 = 0.5 * r * r;
             sfor<1, M + 1>([&](auto m_) {
                 constexpr int m = PK_CI(m_);
-                f = ((2 * m - 1) * f - eT) * it2;
+                f = (2 * m - 1) * f * it2;
                 F[m] = (float)f;
             });
         }

@@ -79,9 +79,11 @@ def test_boys_device_scheme():
     tab = np.array(_arr(src, "pk_boys_table[")).reshape(-1, cols)
     etab = np.array(_arr(src, "pk_boys_exp["))
 
+    xmax = int(re.search(r"#define PK_BOYS_XMAX (\d+)", src).group(1))
+
     def boys(M, T):
         eT = math.exp(-T)
-        if T < 36:
+        if T < xmax:
             i = int(T * 16 + 0.5)
             d = T - i / 16.0
             row = tab[i, M:M + 8]
@@ -97,6 +99,9 @@ def test_boys_device_scheme():
             for m in range(M, 0, -1):
                 out[m - 1] = (2 * T * out[m] + eT) / (2 * m - 1)
             return out
+        # large T: the kernels drop e^{-T} for M <= 4
+        if M <= 4:
+            eT = 0.0
         out = [0.88622692545275801365 * math.sqrt(1.0 / T)]
         for m in range(1, M + 1):
             out.append(((2 * m - 1) * out[-1] - eT) * (0.5 / T))
@@ -104,8 +109,8 @@ def test_boys_device_scheme():
 
     rng = np.random.default_rng(9)
     worst = 0.0
-    for T in list(rng.uniform(0, 40, 120)) + [0.0, 1e-13, 35.9999, 36.0, 80.0, 1e4]:
-        for M in (0, 2, 4, 8, 12):
+    for T in list(rng.uniform(0, 55, 160)) + [0.0, 1e-13, 35.9999, 36.0, 49.99999, 50.0, 50.5, 80.0, 1e4]:
+        for M in (0, 1, 2, 3, 4, 8, 12):
             got = boys(M, float(T))
             for m in range(M + 1):
                 ex = F(m, T)
