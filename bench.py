@@ -40,6 +40,15 @@ WORKLOADS = {
 }
 
 
+def print_classes(cls):
+    """Per-class device time of one profiled build, slowest first (stderr)."""
+    for c in sorted(cls, key=lambda c: -c["ms_avg"]):
+        print("  class (%d%d|%d%d)%s x%d kernels: %8.3f ms  %.3e quartets  %.2f TFLOP/s" % (
+            c["la"], c["lb"], c["lc"], c["ld"], " fp32" if c["precision"] else "", c["kernels"], c["ms_avg"],
+            c["quartets"],
+            c["model_flops"] / (c["ms_avg"] * 1e-3) * 1e-12), file=sys.stderr, flush=True)
+
+
 def bench_water_clusters(args):
     """One step = the J/K build of all 256 clusters: a batch context (merged
     per-class quartet lists), autotuned launch shapes, hipGraph replay.  Beside
@@ -90,6 +99,7 @@ def bench_water_clusters(args):
         c["ms_avg"] = c["ms_total"] / max(c["builds"], 1)
     dom = max(cls, key=lambda c: c["ms_avg"])
     achieved = dom["model_flops"] / (dom["ms_avg"] * 1e-3) * 1e-12
+    print_classes(cls)
     # the same builds one molecule at a time (one context each, eager launches)
     singles = [FockEngine(s, "os", 1e-12) for s in shells]
     Ps = eng.unpack(P.cpu().numpy())
@@ -257,11 +267,7 @@ def main():
     achieved = dom["model_flops"] / (dom["ms_avg"] * 1e-3) * 1e-12
     print("bench: %d steps %.2f ms/step; model %.3e flop/build = %.2f TFLOP/s whole build" % (
         args.steps, ms, st["model_flops"], st["model_flops"] / ms * 1e-9), file=sys.stderr, flush=True)
-    for c in sorted(cls, key=lambda c: -c["ms_avg"]):
-        print("  class (%d%d|%d%d)%s x%d kernels: %8.3f ms  %.3e quartets  %.2f TFLOP/s" % (
-            c["la"], c["lb"], c["lc"], c["ld"], " fp32" if c["precision"] else "", c["kernels"], c["ms_avg"],
-            c["quartets"],
-            c["model_flops"] / (c["ms_avg"] * 1e-3) * 1e-12), file=sys.stderr, flush=True)
+    print_classes(cls)
 
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.workload.replace("-", "_"))
@@ -269,6 +275,24 @@ def main():
         with open(tpath) as f:
             tr = json.load(f)
         traffic = tr.get(dom_name(dom))
+
+    # one-electron integrals (device kernel; the call includes the copy of
+    # S, T, V to the host) and the UHF variant (two builds, device arrays)
+    t = time.perf_counter()
+    mol.one_electron()
+    one_ms = (time.perf_counter() - t) * 1e3
+    Pb = (0.5 * P).contiguous()
+    Jt, Ka, Kb = torch.empty_like(P), torch.empty_like(P), torch.empty_like(P)
+    eng.jk_uhf_device(P.data_ptr(), Pb.data_ptr(), Jt.data_ptr(), Ka.data_ptr(), Kb.data_ptr())
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    nu = max(1, args.steps // 4)
+    for _ in range(nu):
+        eng.jk_uhf_device(P.data_ptr(), Pb.data_ptr(), Jt.data_ptr(), Ka.data_ptr(), Kb.data_ptr())
+    torch.cuda.synchronize()
+    uhf_ms = (time.perf_counter() - t) / nu * 1e3
+    print("bench: one-electron S, T, V %.2f ms (incl. copy to host); UHF J/K %.2f ms per call" % (one_ms, uhf_ms),
+          file=sys.stderr, flush=True)
 
     scf = None
     if not args.no_scf:
@@ -299,6 +323,8 @@ def main():
                      "kernel_ms": dom["ms_avg"], "kernel_model_flops": dom["model_flops"]},
         "cpu_baseline": cpu,
         "scf": scf,
+        "one_electron_ms": round(one_ms, 2),
+        "uhf_ms_per_build": round(uhf_ms, 2),
     }
     print(json.dumps(out), flush=True)
 

@@ -125,6 +125,12 @@ __global__ void k_unpermute(const double *__restrict__ in, double *__restrict__ 
     out[o + i] = in[o + (size_t)ps[r] * nb + ps[c]];
 }
 
+// x += y
+__global__ void k_accumulate(double *__restrict__ x, const double *__restrict__ y, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] += y[i];
+}
+
 __global__ void k_fill(double *__restrict__ x, size_t n, double v) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) x[i] = v;
@@ -213,6 +219,7 @@ struct pk_ctx {
     pk::PrimDev *d_prims = nullptr;
     double *d_Q = nullptr, *d_fnorm = nullptr, *d_P = nullptr, *d_Pn = nullptr, *d_A = nullptr, *d_B = nullptr;
     double *d_J = nullptr, *d_K = nullptr;
+    double *d_J2 = nullptr;  // UHF: J of the second density
     std::vector<ClassWork> work;
     bool work_ok = false;
     int engine = PK_ENGINE_OS;
@@ -926,7 +933,7 @@ int pk_ctx_destroy(pk_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_work(c);
     for (void *p : {(void *)c->d_pairs, (void *)c->d_prims, (void *)c->d_Q, (void *)c->d_fnorm, (void *)c->d_P,
-                    (void *)c->d_Pn, (void *)c->d_A, (void *)c->d_B, (void *)c->d_J, (void *)c->d_K, (void *)c->d_G,
+                    (void *)c->d_Pn, (void *)c->d_A, (void *)c->d_B, (void *)c->d_J, (void *)c->d_K, (void *)c->d_G, (void *)c->d_J2,
                     (void *)c->d_mol_off, (void *)c->d_mol_nb, (void *)c->d_mol_foff, (void *)c->d_fpos})
         if (p) (void)hipFree(p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -1041,18 +1048,38 @@ int pk_fock_2e(pk_ctx *c, const double *P, double *G, int memkind) {
 
 int pk_fock_jk_uhf(pk_ctx *c, const double *Pa, const double *Pb, double *Jt, double *Ka, double *Kb, int memkind) {
     if (!c || !Pa || !Pb) return fail(PK_ERR_ARG, "pk_fock_jk_uhf: null argument");
-    if (memkind != PK_MEM_HOST) return fail(PK_ERR_UNSUPPORTED, "pk_fock_jk_uhf: host pointers only in this build");
-    const size_t n2 = c->n2;
-    std::vector<double> Ja(n2), Jb(n2), ka(n2), kb(n2);
+    if (memkind != PK_MEM_HOST && memkind != PK_MEM_DEVICE) return fail(PK_ERR_ARG, "pk_fock_jk_uhf: bad memkind");
     int rc;
-    if ((rc = pk_fock_jk(c, Pa, Ja.data(), ka.data(), PK_MEM_HOST)) != PK_OK) return rc;
-    if ((rc = pk_fock_jk(c, Pb, Jb.data(), kb.data(), PK_MEM_HOST)) != PK_OK) return rc;
-    for (size_t i = 0; i < n2; ++i) {
-        if (Jt) Jt[i] = Ja[i] + Jb[i];
-        if (Ka) Ka[i] = ka[i];
-        if (Kb) Kb[i] = kb[i];
+    if ((rc = ensure_buffers(c)) != PK_OK) return rc;
+    const size_t n2 = c->n2 * sizeof(double);
+    if (!c->d_J2) PK_HIP(hipMalloc((void **)&c->d_J2, n2));
+    hipStream_t s = c->stream;
+    const unsigned g = (unsigned)((c->n2 + 255) / 256);
+    // two builds on the device: (J[Pa], K[Pa]) and (J[Pb], K[Pb]); Jt = J[Pa] + J[Pb]
+    if (memkind == PK_MEM_DEVICE) {
+        if ((rc = build_device(c, Pa, Jt ? Jt : nullptr, Ka, nullptr)) != PK_OK) return rc;
+        if ((rc = build_device(c, Pb, Jt ? c->d_J2 : nullptr, Kb, nullptr)) != PK_OK) return rc;
+        if (Jt) {
+            hipLaunchKernelGGL(k_accumulate, dim3(g), dim3(256), 0, s, Jt, c->d_J2, c->n2);
+            PK_HIP(hipGetLastError());
+        }
+        PK_HIP(hipStreamSynchronize(s));
+        return finish_timing(c);
     }
-    return PK_OK;
+    // host memory: the context's buffers (a captured build writes d_J / d_K)
+    PK_HIP(hipMemcpyAsync(c->d_P, Pa, n2, hipMemcpyHostToDevice, s));
+    if ((rc = build_device(c, c->d_P, c->d_J, c->d_K, nullptr)) != PK_OK) return rc;
+    if (Ka) PK_HIP(hipMemcpyAsync(Ka, c->d_K, n2, hipMemcpyDeviceToHost, s));
+    PK_HIP(hipMemcpyAsync(c->d_J2, c->d_J, n2, hipMemcpyDeviceToDevice, s));
+    PK_HIP(hipStreamSynchronize(s));
+    PK_HIP(hipMemcpyAsync(c->d_P, Pb, n2, hipMemcpyHostToDevice, s));
+    if ((rc = build_device(c, c->d_P, c->d_J, c->d_K, nullptr)) != PK_OK) return rc;
+    hipLaunchKernelGGL(k_accumulate, dim3(g), dim3(256), 0, s, c->d_J, c->d_J2, c->n2);
+    PK_HIP(hipGetLastError());
+    if (Jt) PK_HIP(hipMemcpyAsync(Jt, c->d_J, n2, hipMemcpyDeviceToHost, s));
+    if (Kb) PK_HIP(hipMemcpyAsync(Kb, c->d_K, n2, hipMemcpyDeviceToHost, s));
+    PK_HIP(hipStreamSynchronize(s));
+    return finish_timing(c);
 }
 
 int pk_schwarz(pk_ctx *c, double *Q, int memkind) {

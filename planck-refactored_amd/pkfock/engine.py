@@ -215,6 +215,21 @@ class FockEngine:
         _check(load().pk_fock_jk(self.ctx, _ptr(P), _ptr(J), _ptr(K), MEM_HOST), "pk_fock_jk")
         return J, K
 
+    def jk_uhf(self, Pa, Pb):
+        """UHF: (J[Pa + Pb], K[Pa], K[Pb]) (pk_fock_jk_uhf, host arrays)."""
+        Pa = np.ascontiguousarray(Pa, dtype=np.float64)
+        Pb = np.ascontiguousarray(Pb, dtype=np.float64)
+        Jt, Ka, Kb = np.zeros_like(Pa), np.zeros_like(Pa), np.zeros_like(Pa)
+        _check(load().pk_fock_jk_uhf(self.ctx, _ptr(Pa), _ptr(Pb), _ptr(Jt), _ptr(Ka), _ptr(Kb), MEM_HOST),
+               "pk_fock_jk_uhf")
+        return Jt, Ka, Kb
+
+    def jk_uhf_device(self, Pa_ptr, Pb_ptr, Jt_ptr, Ka_ptr, Kb_ptr):
+        """Device pointers (ints) for Pa, Pb, Jt, Ka, Kb."""
+        v = ctypes.c_void_p
+        _check(load().pk_fock_jk_uhf(self.ctx, v(Pa_ptr), v(Pb_ptr), v(Jt_ptr), v(Ka_ptr), v(Kb_ptr), MEM_DEVICE),
+               "pk_fock_jk_uhf")
+
     def g(self, P):
         P = np.ascontiguousarray(P, dtype=np.float64)
         G = np.zeros_like(P)
