@@ -43,10 +43,10 @@ WORKLOADS = {
 def print_classes(cls):
     """Per-class device time of one profiled build, slowest first (stderr)."""
     for c in sorted(cls, key=lambda c: -c["ms_avg"]):
-        print("  class (%d%d|%d%d)%s x%d kernels: %8.3f ms  %.3e quartets  %.2f TFLOP/s" % (
+        print("  class (%d%d|%d%d)%s x%d kernels: %8.3f ms  %.3e quartets  %.2f TFLOP/s  %.1f kets/wave" % (
             c["la"], c["lb"], c["lc"], c["ld"], " fp32" if c["precision"] else "", c["kernels"], c["ms_avg"],
-            c["quartets"],
-            c["model_flops"] / (c["ms_avg"] * 1e-3) * 1e-12), file=sys.stderr, flush=True)
+            c["quartets"], c["model_flops"] / (c["ms_avg"] * 1e-3) * 1e-12,
+            c["quartets"] / max(c.get("waves", 0), 1)), file=sys.stderr, flush=True)
 
 
 def bench_water_clusters(args):

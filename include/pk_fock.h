@@ -151,6 +151,7 @@ typedef struct pk_class_stats {
     int32_t precision;            /* PK_PREC_FP64, or PK_PREC_MIXED for the FP32 list */
     int32_t block;                /* lanes per workgroup of the class kernels */
     int32_t remap;                /* 1: XCD-contiguous wave ranges */
+    int64_t waves;                /* work items (one bra, up to 64 kets per wave) */
 } pk_class_stats;
 int pk_ctx_profile(pk_ctx *ctx, int enable);
 int pk_ctx_class_count(const pk_ctx *ctx);

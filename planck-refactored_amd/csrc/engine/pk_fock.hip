@@ -1008,6 +1008,7 @@ int pk_ctx_class_stats(const pk_ctx *c, int i, pk_class_stats *o) {
     o->precision = w.prec;
     o->block = w.block;
     o->remap = w.remap;
+    o->waves = w.nwaves;
     return PK_OK;
 }
 

@@ -42,7 +42,8 @@ class ClassStats(ctypes.Structure):
     _fields_ = [("la", ctypes.c_int32), ("lb", ctypes.c_int32), ("lc", ctypes.c_int32), ("ld", ctypes.c_int32),
                 ("kernels", ctypes.c_int32), ("quartets", ctypes.c_int64), ("primitive_quartets", ctypes.c_int64),
                 ("model_flops", ctypes.c_double), ("ms_total", ctypes.c_double), ("builds", ctypes.c_int64),
-                ("precision", ctypes.c_int32), ("block", ctypes.c_int32), ("remap", ctypes.c_int32)]
+                ("precision", ctypes.c_int32), ("block", ctypes.c_int32), ("remap", ctypes.c_int32),
+                ("waves", ctypes.c_int64)]
 
 
 _lib = None
