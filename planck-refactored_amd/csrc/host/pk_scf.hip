@@ -662,10 +662,22 @@ int pk_scf_rhf_batch(const pk_basis *const *bases, int nmol, const pk_scf_option
         std::vector<double *> Fh, Eh;
         std::deque<int> slots;
         std::vector<std::vector<double>> Bm;
-        double Eprev = 0.0;
+        double Eprev = 0.0, Etot = 0.0;
+        const double *Fd = nullptr;  // Fock matrix (DIIS-extrapolated) of this iteration
         bool done = false;
     };
     std::vector<Mol> mols(nmol);
+    // molecules of one size and occupation: their density steps run as one
+    // batched call sequence (rocBLAS / rocSOLVER batched over pointer arrays)
+    struct Group {
+        int n = 0, nocc = 0;
+        std::vector<int> members;
+        double *D = nullptr, *E = nullptr;
+        rocblas_int *info = nullptr;
+        double **ptr = nullptr;          // device: 6 arrays of members.size() pointers
+        std::vector<double *> hptr;      // host copy
+    };
+    std::vector<Group> groups;
     Dev dev;
     hipStream_t st = nullptr;
     rocblas_handle hb = nullptr;
@@ -705,9 +717,9 @@ int pk_scf_rhf_batch(const pk_basis *const *bases, int nmol, const pk_scf_option
     auto density = [&](Mol &m, const double *Fm, double *Pt) -> int { return density_h(hb, m, Fm, Pt); };
 
     // one SCF iteration of one molecule after the batched build (the body of
-    // pk_scf_rhf's iteration), on stream s2 with handle h
-    auto step = [&](rocblas_handle h, hipStream_t s2, Mol &m, pk_scf_result &r, int it, double build_ms,
-                    std::vector<double> &hpart) -> int {
+    // pk_scf_rhf's iteration up to the DIIS-extrapolated Fock matrix), on
+    // stream s2 with handle h
+    auto step = [&](rocblas_handle h, hipStream_t s2, Mol &m, pk_scf_result &r, int it, double build_ms) -> int {
 #define BSTEP_RB(x)                                                                             \
     do {                                                                                        \
         if ((x) != rocblas_status_success) return pkh::fail(PK_ERR_HIP, #x " failed");          \
@@ -760,8 +772,15 @@ int pk_scf_rhf_batch(const pk_basis *const *bases, int nmol, const pk_scf_option
             BSTEP_HIP(hipMemcpyAsync(m.D, m.M, n2 * 8, hipMemcpyDeviceToDevice, s2));
             Fd = m.D;
         }
-        const int rc3 = density_h(h, m, Fd, m.Pn);
-        if (rc3 != PK_OK) return rc3;
+        m.Fd = Fd;
+        m.Etot = Etot;
+        BSTEP_HIP(hipStreamSynchronize(s2));
+        return PK_OK;
+    };
+    // after the batched density step: the change of the density, convergence
+    auto step_b = [&](hipStream_t s2, Mol &m, pk_scf_result &r, int it, std::vector<double> &hpart) -> int {
+        const size_t n2 = m.n2;
+        const double Etot = m.Etot;
         hipLaunchKernelGGL(k_diff, dim3(nparts), dim3(256), 0, s2, m.Pn, m.P, n2, m.part);
         BSTEP_HIP(hipMemcpyAsync(hpart.data(), m.part, 2 * nparts * 8, hipMemcpyDeviceToHost, s2));
         BSTEP_HIP(hipMemcpyAsync(m.P, m.Pn, n2 * 8, hipMemcpyDeviceToDevice, s2));
@@ -864,10 +883,10 @@ int pk_scf_rhf_batch(const pk_basis *const *bases, int nmol, const pk_scf_option
             goto done;
         }
     }
-    // workers: the per-molecule algebra of an iteration runs on up to 8 host
+    // workers: the per-molecule algebra of an iteration runs on up to 16 host
     // threads, each with its own stream and rocBLAS handle (the small dense
     // calls are launch-bound); the batched build stays on the main stream
-    nworkers = std::max(1, std::min(8, nmol));
+    nworkers = std::max(1, std::min(16, nmol));
     wst.assign(nworkers, nullptr);
     whb.assign(nworkers, nullptr);
     for (int w = 0; w < nworkers; ++w) {
@@ -875,6 +894,33 @@ int pk_scf_rhf_batch(const pk_basis *const *bases, int nmol, const pk_scf_option
         SCF_RB(rocblas_create_handle(&whb[w]));
         SCF_RB(rocblas_set_stream(whb[w], wst[w]));
         SCF_RB(rocblas_set_pointer_mode(whb[w], rocblas_pointer_mode_host));
+    }
+    // size groups for the batched density step
+    {
+        std::map<std::pair<int, int>, int> gi;
+        for (int i = 0; i < nmol; ++i) {
+            const auto key = std::make_pair(mols[i].n, mols[i].nocc);
+            auto f = gi.find(key);
+            if (f == gi.end()) {
+                f = gi.emplace(key, (int)groups.size()).first;
+                groups.emplace_back();
+                groups.back().n = mols[i].n;
+                groups.back().nocc = mols[i].nocc;
+            }
+            groups[f->second].members.push_back(i);
+        }
+        for (Group &g : groups) {
+            const size_t cnt = g.members.size();
+            g.D = dev.alloc((size_t)g.n * cnt);
+            g.E = dev.alloc((size_t)g.n * cnt);
+            g.info = (rocblas_int *)dev.alloc(cnt);
+            g.ptr = (double **)dev.alloc(6 * cnt);
+            g.hptr.resize(6 * cnt);
+            if (!g.D || !g.E || !g.info || !g.ptr) {
+                rc = pkh::fail(PK_ERR_ALLOC, "pk_scf_rhf_batch: device allocation failed");
+                goto done;
+            }
+        }
     }
     for (int it = 1; it <= max_cycles; ++it) {
         SCF_PK(pk_fock_2e(ctx, Pp, Gp, PK_MEM_DEVICE));
@@ -885,33 +931,70 @@ int pk_scf_rhf_batch(const pk_basis *const *bases, int nmol, const pk_scf_option
             build_ms = s.last_build_ms;
         }
         SCF_HIP(hipStreamSynchronize(st));
-        std::atomic<int> next{0};
         std::vector<int> wrc(nworkers, PK_OK);
         std::vector<std::string> werr(nworkers);
-        auto worker = [&](int w) {
-            rocblas_handle h = whb[w];
-            hipStream_t s2 = wst[w];
-            std::vector<double> hpart(2 * nparts);
-            for (int i; (i = next.fetch_add(1)) < nmol;) {
-                Mol &m = mols[i];
-                if (m.done) continue;
-                const int rc2 = step(h, s2, m, res[i], it, build_ms, hpart);
-                if (rc2 != PK_OK) {
-                    wrc[w] = rc2;
-                    werr[w] = pkh::g_err;
-                    return;
+        // run fn(worker, molecule) for every molecule not yet converged
+        auto run = [&](auto fn) {
+            std::atomic<int> next{0};
+            auto worker = [&](int w) {
+                for (int i; (i = next.fetch_add(1)) < nmol;) {
+                    if (mols[i].done) continue;
+                    const int rc2 = fn(w, i);
+                    if (rc2 != PK_OK) {
+                        wrc[w] = rc2;
+                        werr[w] = pkh::g_err;
+                        return;
+                    }
                 }
-            }
+            };
+            std::vector<std::thread> th;
+            for (int w = 1; w < nworkers; ++w) th.emplace_back(worker, w);
+            worker(0);
+            for (std::thread &t : th) t.join();
+            for (int w = 0; w < nworkers; ++w)
+                if (wrc[w] != PK_OK) return pkh::fail(wrc[w], werr[w]);
+            return (int)PK_OK;
         };
-        std::vector<std::thread> th;
-        for (int w = 1; w < nworkers; ++w) th.emplace_back(worker, w);
-        worker(0);
-        for (std::thread &t : th) t.join();
-        for (int w = 0; w < nworkers; ++w)
-            if (wrc[w] != PK_OK) {
-                rc = pkh::fail(wrc[w], werr[w]);
-                goto done;
+        // (1) per molecule: Fock matrix, energy, DIIS error and extrapolation
+        if ((rc = run([&](int w, int i) { return step(whb[w], wst[w], mols[i], res[i], it, build_ms); })) != PK_OK)
+            goto done;
+        // (2) per size group, batched: P_new = 2 C_occ C_occ^T, C = X eig(X F X)
+        for (Group &g : groups) {
+            std::vector<int> act;
+            for (int i : g.members)
+                if (!mols[i].done) act.push_back(i);
+            const int cnt = (int)act.size(), n = g.n;
+            if (cnt == 0) continue;
+            for (int k = 0; k < cnt; ++k) {
+                const Mol &m = mols[act[k]];
+                g.hptr[0 * cnt + k] = m.X;
+                g.hptr[1 * cnt + k] = const_cast<double *>(m.Fd);
+                g.hptr[2 * cnt + k] = m.M;
+                g.hptr[3 * cnt + k] = m.W1;
+                g.hptr[4 * cnt + k] = m.C;
+                g.hptr[5 * cnt + k] = m.Pn;
             }
+            SCF_HIP(hipMemcpyAsync(g.ptr, g.hptr.data(), 6 * cnt * sizeof(double *), hipMemcpyHostToDevice, st));
+            double *const *pX = g.ptr, *const *pF = g.ptr + cnt, *const *pM = g.ptr + 2 * cnt,
+                          *const *pW = g.ptr + 3 * cnt, *const *pC = g.ptr + 4 * cnt, *const *pP = g.ptr + 5 * cnt;
+            const rocblas_operation N = rocblas_operation_none, T = rocblas_operation_transpose;
+            SCF_RB(rocblas_dgemm_batched(hb, N, N, n, n, n, &one, pX, n, pF, n, &zero, pM, n, cnt));
+            SCF_RB(rocblas_dgemm_batched(hb, N, N, n, n, n, &one, pM, n, pX, n, &zero, pW, n, cnt));
+            SCF_RB(rocsolver_dsyevd_batched(hb, rocblas_evect_original, rocblas_fill_upper, n, pW, n, g.D, n, g.E, n,
+                                            g.info, cnt));
+            SCF_RB(rocblas_dgemm_batched(hb, N, N, n, n, n, &one, pX, n, pW, n, &zero, pC, n, cnt));
+            SCF_RB(rocblas_dgemm_batched(hb, N, T, n, n, g.nocc, &two, pC, n, pC, n, &zero, pP, n, cnt));
+        }
+        // the new densities are read on the workers' streams; the pointer
+        // tables are rewritten next iteration
+        SCF_HIP(hipStreamSynchronize(st));
+        // (3) per molecule: density change, convergence
+        if ((rc = run([&](int w, int i) {
+                 thread_local std::vector<double> hpart;
+                 hpart.resize(2 * nparts);
+                 return step_b(wst[w], mols[i], res[i], it, hpart);
+             })) != PK_OK)
+            goto done;
         bool all = true;
         for (const Mol &m : mols) all = all && m.done;
         if (all) break;
