@@ -17,6 +17,7 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <cstring>
 #include <deque>
 #include <map>
 #include <string>
@@ -69,6 +70,111 @@ __global__ void k_diff(const double *__restrict__ a, const double *__restrict__ 
     if (threadIdx.x == 0) {
         out[2 * blockIdx.x] = smax[0];
         out[2 * blockIdx.x + 1] = ssum[0];
+    }
+}
+
+// ---- molecule-table kernels of the batched SCF: blockIdx.y selects an entry
+// of the tables (one molecule, or one molecule and DIIS slot), blockIdx.x and
+// the threads stride over its n x n elements; reductions leave one partial
+// per block, summed on the host.
+__device__ __forceinline__ void block_sum2(double a, double b, double *out) {
+    __shared__ double sa[256], sb[256];
+    sa[threadIdx.x] = a;
+    sb[threadIdx.x] = b;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            sa[threadIdx.x] += sa[threadIdx.x + o];
+            sb[threadIdx.x] += sb[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = sa[0];
+        out[1] = sb[0];
+    }
+}
+
+// F = H + G; partials of sum P.H and sum P.F
+__global__ void kb_fock(const double *const *H, const double *const *G, double *const *F, const double *const *P,
+                        const long long *n2, double *part) {
+    const int y = blockIdx.y;
+    const size_t n = (size_t)n2[y];
+    const double *h = H[y], *g = G[y], *p = P[y];
+    double *f = F[y];
+    double eh = 0.0, ef = 0.0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const double fi = h[i] + g[i];
+        f[i] = fi;
+        eh = fma(p[i], h[i], eh);
+        ef = fma(p[i], fi, ef);
+    }
+    block_sum2(eh, ef, part + 2 * ((size_t)y * gridDim.x + blockIdx.x));
+}
+
+// D = M - M^T (column-major n x n)
+__global__ void kb_antisym(const double *const *M, double *const *D, const int *nn) {
+    const int y = blockIdx.y, n = nn[y];
+    const double *m = M[y];
+    double *d = D[y];
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < (size_t)n * n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const int r = (int)(i % n), c = (int)(i / n);
+        d[i] = m[i] - m[(size_t)r * n + c];
+    }
+}
+
+// partials of a . b (second partial unused)
+__global__ void kb_dots(const double *const *A, const double *const *B, const long long *n2, double *part) {
+    const int y = blockIdx.y;
+    const size_t n = (size_t)n2[y];
+    const double *a = A[y], *b = B[y];
+    double s = 0.0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        s = fma(a[i], b[i], s);
+    block_sum2(s, 0.0, part + 2 * ((size_t)y * gridDim.x + blockIdx.x));
+}
+
+// out = sum_k c[kmax y + k] src[kmax y + k], k < ns[y]
+__global__ void kb_combine(const double *const *src, const double *coef, const int *ns, int kmax,
+                           double *const *out, const long long *n2) {
+    const int y = blockIdx.y, k1 = ns[y];
+    const size_t n = (size_t)n2[y];
+    double *o = out[y];
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        double v = 0.0;
+        for (int k = 0; k < k1; ++k) v = fma(coef[(size_t)kmax * y + k], src[(size_t)kmax * y + k][i], v);
+        o[i] = v;
+    }
+}
+
+// partials of max |Pn - P| and sum (Pn - P)^2, then P = Pn
+__global__ void kb_diff(const double *const *Pn, double *const *P, const long long *n2, double *part) {
+    __shared__ double smax[256], ssum[256];
+    const int y = blockIdx.y;
+    const size_t n = (size_t)n2[y];
+    const double *a = Pn[y];
+    double *b = P[y];
+    double m = 0.0, s = 0.0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const double d = a[i] - b[i];
+        m = fmax(m, fabs(d));
+        s = fma(d, d, s);
+        b[i] = a[i];
+    }
+    smax[threadIdx.x] = m;
+    ssum[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + o]);
+            ssum[threadIdx.x] += ssum[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * ((size_t)y * gridDim.x + blockIdx.x)] = smax[0];
+        part[2 * ((size_t)y * gridDim.x + blockIdx.x) + 1] = ssum[0];
     }
 }
 
@@ -674,8 +780,6 @@ int pk_scf_rhf_batch(const pk_basis *const *bases, int nmol, const pk_scf_option
         std::vector<int> members;
         double *D = nullptr, *E = nullptr;
         rocblas_int *info = nullptr;
-        double **ptr = nullptr;          // device: 6 arrays of members.size() pointers
-        std::vector<double *> hptr;      // host copy
     };
     std::vector<Group> groups;
     Dev dev;
@@ -693,9 +797,6 @@ int pk_scf_rhf_batch(const pk_basis *const *bases, int nmol, const pk_scf_option
     size_t ntot = 0;
     double *Pp = nullptr, *Gp = nullptr;
     (void)one;
-    int nworkers = 0;
-    std::vector<hipStream_t> wst;
-    std::vector<rocblas_handle> whb;
     auto gemm_h = [&](rocblas_handle h, Mol &m, rocblas_operation ta, rocblas_operation tb, int k, const double *A,
                       const double *B, double *Cm) {
         return rocblas_dgemm(h, ta, tb, m.n, m.n, k, &one, A, m.n, B, m.n, &zero, Cm, m.n);
@@ -716,92 +817,6 @@ int pk_scf_rhf_batch(const pk_basis *const *bases, int nmol, const pk_scf_option
     };
     auto density = [&](Mol &m, const double *Fm, double *Pt) -> int { return density_h(hb, m, Fm, Pt); };
 
-    // one SCF iteration of one molecule after the batched build (the body of
-    // pk_scf_rhf's iteration up to the DIIS-extrapolated Fock matrix), on
-    // stream s2 with handle h
-    auto step = [&](rocblas_handle h, hipStream_t s2, Mol &m, pk_scf_result &r, int it, double build_ms) -> int {
-#define BSTEP_RB(x)                                                                             \
-    do {                                                                                        \
-        if ((x) != rocblas_status_success) return pkh::fail(PK_ERR_HIP, #x " failed");          \
-    } while (0)
-#define BSTEP_HIP(x)                                                                            \
-    do {                                                                                        \
-        hipError_t e_ = (x);                                                                    \
-        if (e_ != hipSuccess) return pkh::fail(PK_ERR_HIP, std::string(#x ": ") + hipGetErrorString(e_)); \
-    } while (0)
-        const int n = m.n, g2 = (int)((m.n2 + 255) / 256);
-        const size_t n2 = m.n2;
-        r.fock_ms += build_ms;
-        hipLaunchKernelGGL(k_add, dim3(g2), dim3(256), 0, s2, m.H, m.G, m.F, n2);
-        double eph = 0, epf = 0;
-        BSTEP_RB(rocblas_ddot(h, (rocblas_int)n2, m.P, 1, m.H, 1, &eph));
-        BSTEP_RB(rocblas_ddot(h, (rocblas_int)n2, m.P, 1, m.F, 1, &epf));
-        const double Etot = 0.5 * (eph + epf) + r.nuclear_repulsion;
-        BSTEP_RB(gemm_h(h, m, rocblas_operation_none, rocblas_operation_none, n, m.F, m.P, m.W1));
-        BSTEP_RB(gemm_h(h, m, rocblas_operation_none, rocblas_operation_none, n, m.W1, m.S, m.M));
-        hipLaunchKernelGGL(k_antisym, dim3(g2), dim3(256), 0, s2, m.M, m.D, n);
-        BSTEP_RB(gemm_h(h, m, rocblas_operation_none, rocblas_operation_none, n, m.X, m.D, m.W1));
-        int slot;
-        if ((int)m.slots.size() < diis_dim) {
-            slot = (int)m.slots.size();
-        } else {
-            slot = m.slots.front();
-            m.slots.pop_front();
-        }
-        BSTEP_RB(gemm_h(h, m, rocblas_operation_none, rocblas_operation_none, n, m.W1, m.X, m.Eh[slot]));
-        BSTEP_HIP(hipMemcpyAsync(m.Fh[slot], m.F, n2 * 8, hipMemcpyDeviceToDevice, s2));
-        m.slots.push_back(slot);
-        for (int s3 : m.slots) {
-            double v = 0;
-            BSTEP_RB(rocblas_ddot(h, (rocblas_int)n2, m.Eh[slot], 1, m.Eh[s3], 1, &v));
-            m.Bm[slot][s3] = m.Bm[s3][slot] = v;
-        }
-        if (it == 1) r.first_diis_error = std::sqrt(m.Bm[slot][slot] / (double)n2);
-        const double *Fd = m.F;
-        if (m.slots.size() >= 2) {
-            const int k = (int)m.slots.size();
-            std::vector<double> A((size_t)(k + 1) * (k + 1), 0.0), rhs(k + 1, 0.0);
-            for (int a2 = 0; a2 < k; ++a2) {
-                for (int b2 = 0; b2 < k; ++b2) A[(size_t)a2 * (k + 1) + b2] = m.Bm[m.slots[a2]][m.slots[b2]];
-                A[(size_t)a2 * (k + 1) + k] = A[(size_t)k * (k + 1) + a2] = -1.0;
-            }
-            rhs[k] = -1.0;
-            const std::vector<double> cf = qr_solve(k + 1, A, rhs);
-            BSTEP_HIP(hipMemsetAsync(m.M, 0, n2 * 8, s2));
-            for (int a2 = 0; a2 < k; ++a2) BSTEP_RB(rocblas_daxpy(h, (rocblas_int)n2, &cf[a2], m.Fh[m.slots[a2]], 1, m.M, 1));
-            BSTEP_HIP(hipMemcpyAsync(m.D, m.M, n2 * 8, hipMemcpyDeviceToDevice, s2));
-            Fd = m.D;
-        }
-        m.Fd = Fd;
-        m.Etot = Etot;
-        BSTEP_HIP(hipStreamSynchronize(s2));
-        return PK_OK;
-    };
-    // after the batched density step: the change of the density, convergence
-    auto step_b = [&](hipStream_t s2, Mol &m, pk_scf_result &r, int it, std::vector<double> &hpart) -> int {
-        const size_t n2 = m.n2;
-        const double Etot = m.Etot;
-        hipLaunchKernelGGL(k_diff, dim3(nparts), dim3(256), 0, s2, m.Pn, m.P, n2, m.part);
-        BSTEP_HIP(hipMemcpyAsync(hpart.data(), m.part, 2 * nparts * 8, hipMemcpyDeviceToHost, s2));
-        BSTEP_HIP(hipMemcpyAsync(m.P, m.Pn, n2 * 8, hipMemcpyDeviceToDevice, s2));
-        BSTEP_HIP(hipStreamSynchronize(s2));
-        double dmax = 0, dss = 0;
-        for (int k = 0; k < nparts; ++k) {
-            dmax = std::max(dmax, hpart[2 * k]);
-            dss += hpart[2 * k + 1];
-        }
-        const double drms = std::sqrt(dss / (double)n2), dE = std::fabs(Etot - m.Eprev);
-        m.Eprev = Etot;
-        r.iterations = it;
-        r.energy = Etot;
-        if (it > 1 && dE < opt->tol_energy && drms < opt->tol_density && dmax < opt->tol_density) {
-            r.converged = 1;
-            m.done = true;
-        }
-        return PK_OK;
-#undef BSTEP_RB
-#undef BSTEP_HIP
-    };
 
     SCF_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     SCF_RB(rocblas_create_handle(&hb));
@@ -883,18 +898,6 @@ int pk_scf_rhf_batch(const pk_basis *const *bases, int nmol, const pk_scf_option
             goto done;
         }
     }
-    // workers: the per-molecule algebra of an iteration runs on up to 16 host
-    // threads, each with its own stream and rocBLAS handle (the small dense
-    // calls are launch-bound); the batched build stays on the main stream
-    nworkers = std::max(1, std::min(16, nmol));
-    wst.assign(nworkers, nullptr);
-    whb.assign(nworkers, nullptr);
-    for (int w = 0; w < nworkers; ++w) {
-        SCF_HIP(hipStreamCreateWithFlags(&wst[w], hipStreamNonBlocking));
-        SCF_RB(rocblas_create_handle(&whb[w]));
-        SCF_RB(rocblas_set_stream(whb[w], wst[w]));
-        SCF_RB(rocblas_set_pointer_mode(whb[w], rocblas_pointer_mode_host));
-    }
     // size groups for the batched density step
     {
         std::map<std::pair<int, int>, int> gi;
@@ -914,90 +917,295 @@ int pk_scf_rhf_batch(const pk_basis *const *bases, int nmol, const pk_scf_option
             g.D = dev.alloc((size_t)g.n * cnt);
             g.E = dev.alloc((size_t)g.n * cnt);
             g.info = (rocblas_int *)dev.alloc(cnt);
-            g.ptr = (double **)dev.alloc(6 * cnt);
-            g.hptr.resize(6 * cnt);
-            if (!g.D || !g.E || !g.info || !g.ptr) {
+            if (!g.D || !g.E || !g.info) {
                 rc = pkh::fail(PK_ERR_ALLOC, "pk_scf_rhf_batch: device allocation failed");
                 goto done;
             }
         }
     }
-    for (int it = 1; it <= max_cycles; ++it) {
-        SCF_PK(pk_fock_2e(ctx, Pp, Gp, PK_MEM_DEVICE));
-        double build_ms = 0;
-        {
-            pk_stats s;
-            pk_ctx_stats(ctx, &s);
-            build_ms = s.last_build_ms;
-        }
-        SCF_HIP(hipStreamSynchronize(st));
-        std::vector<int> wrc(nworkers, PK_OK);
-        std::vector<std::string> werr(nworkers);
-        // run fn(worker, molecule) for every molecule not yet converged
-        auto run = [&](auto fn) {
-            std::atomic<int> next{0};
-            auto worker = [&](int w) {
-                for (int i; (i = next.fetch_add(1)) < nmol;) {
-                    if (mols[i].done) continue;
-                    const int rc2 = fn(w, i);
-                    if (rc2 != PK_OK) {
-                        wrc[w] = rc2;
-                        werr[w] = pkh::g_err;
-                        return;
-                    }
-                }
-            };
-            std::vector<std::thread> th;
-            for (int w = 1; w < nworkers; ++w) th.emplace_back(worker, w);
-            worker(0);
-            for (std::thread &t : th) t.join();
-            for (int w = 0; w < nworkers; ++w)
-                if (wrc[w] != PK_OK) return pkh::fail(wrc[w], werr[w]);
-            return (int)PK_OK;
-        };
-        // (1) per molecule: Fock matrix, energy, DIIS error and extrapolation
-        if ((rc = run([&](int w, int i) { return step(whb[w], wst[w], mols[i], res[i], it, build_ms); })) != PK_OK)
+    // Iterations.  Every step of every unconverged molecule runs as a few
+    // launches over molecule tables (pointer arrays and sizes staged in one
+    // host buffer and copied in one transfer): F = H + G with the energy
+    // partial sums, the DIIS error Xt (F P S - S P F) X (batched GEMMs per
+    // size group), the new row of each molecule's DIIS matrix, then, after
+    // the host's small DIIS solves, the extrapolated Fock matrices, the
+    // batched density step and the density change (two host synchronisations
+    // per iteration).
+    {
+        const int nblk = 32;
+        // per molecule: phase A 17 slots + 3 per DIIS vector, phase B 12 + 2 per DIIS vector
+        const size_t stage_bytes = (size_t)nmol * 8 * (32 + 4 * (size_t)diis_dim) + 4096;
+        char *dstage = (char *)dev.alloc(stage_bytes / 8);
+        // partials: 2 per block for each molecule and each DIIS matrix entry
+        const size_t npart = (size_t)nmol * 2 * nblk * (2 + (size_t)diis_dim);
+        double *dpart = dev.alloc(npart);
+        if (!dstage || !dpart) {
+            rc = pkh::fail(PK_ERR_ALLOC, "pk_scf_rhf_batch: device allocation failed");
             goto done;
-        // (2) per size group, batched: P_new = 2 C_occ C_occ^T, C = X eig(X F X)
-        for (Group &g : groups) {
-            std::vector<int> act;
-            for (int i : g.members)
-                if (!mols[i].done) act.push_back(i);
-            const int cnt = (int)act.size(), n = g.n;
-            if (cnt == 0) continue;
-            for (int k = 0; k < cnt; ++k) {
-                const Mol &m = mols[act[k]];
-                g.hptr[0 * cnt + k] = m.X;
-                g.hptr[1 * cnt + k] = const_cast<double *>(m.Fd);
-                g.hptr[2 * cnt + k] = m.M;
-                g.hptr[3 * cnt + k] = m.W1;
-                g.hptr[4 * cnt + k] = m.C;
-                g.hptr[5 * cnt + k] = m.Pn;
+        }
+        std::vector<char> hstage(stage_bytes);
+        std::vector<double> hpart(npart);
+        size_t used = 0;
+        bool overflow = false;
+        // place `bytes` at the next 8-byte slot of the staging buffer; returns
+        // the device address it will have
+        auto put = [&](const void *src, size_t bytes) -> void * {
+            const size_t at = used;
+            if (at + bytes > hstage.size()) {
+                overflow = true;
+                return dstage;
             }
-            SCF_HIP(hipMemcpyAsync(g.ptr, g.hptr.data(), 6 * cnt * sizeof(double *), hipMemcpyHostToDevice, st));
-            double *const *pX = g.ptr, *const *pF = g.ptr + cnt, *const *pM = g.ptr + 2 * cnt,
-                          *const *pW = g.ptr + 3 * cnt, *const *pC = g.ptr + 4 * cnt, *const *pP = g.ptr + 5 * cnt;
-            const rocblas_operation N = rocblas_operation_none, T = rocblas_operation_transpose;
-            SCF_RB(rocblas_dgemm_batched(hb, N, N, n, n, n, &one, pX, n, pF, n, &zero, pM, n, cnt));
-            SCF_RB(rocblas_dgemm_batched(hb, N, N, n, n, n, &one, pM, n, pX, n, &zero, pW, n, cnt));
-            SCF_RB(rocsolver_dsyevd_batched(hb, rocblas_evect_original, rocblas_fill_upper, n, pW, n, g.D, n, g.E, n,
-                                            g.info, cnt));
-            SCF_RB(rocblas_dgemm_batched(hb, N, N, n, n, n, &one, pX, n, pW, n, &zero, pC, n, cnt));
-            SCF_RB(rocblas_dgemm_batched(hb, N, T, n, n, g.nocc, &two, pC, n, pC, n, &zero, pP, n, cnt));
+            std::memcpy(hstage.data() + at, src, bytes);
+            used += (bytes + 7) / 8 * 8;
+            return dstage + at;
+        };
+        const dim3 blk(256);
+        for (int it = 1; it <= max_cycles; ++it) {
+            SCF_PK(pk_fock_2e(ctx, Pp, Gp, PK_MEM_DEVICE));
+            double build_ms = 0;
+            {
+                pk_stats s2;
+                pk_ctx_stats(ctx, &s2);
+                build_ms = s2.last_build_ms;
+            }
+            std::vector<int> act;
+            for (int i = 0; i < nmol; ++i)
+                if (!mols[i].done) act.push_back(i);
+            const int cnt = (int)act.size();
+            if (cnt == 0) break;
+            const rocblas_operation N = rocblas_operation_none;
+            // molecule tables of one size group (device pointer arrays)
+            struct GT {
+                int n, cnt;
+                double *const *t[8];
+                const Group *grp;
+            };
+            // ---- phase A (its device tables live in the staging buffer until
+            // phase B rewrites it: they are scoped to the block below)
+            used = 0;
+            std::vector<int> slot(nmol, -1);
+            for (int i : act) {
+                Mol &m = mols[i];
+                if ((int)m.slots.size() < diis_dim) {
+                    slot[i] = (int)m.slots.size();
+                } else {
+                    slot[i] = m.slots.front();
+                    m.slots.pop_front();
+                }
+                m.slots.push_back(slot[i]);
+            }
+            std::vector<std::pair<int, int>> pairs;  // DIIS matrix entries: (molecule, slot)
+            int np2 = 0;
+            {
+            std::vector<const double *> vH, vG, vP;
+            std::vector<double *> vF;
+            std::vector<long long> vn2;
+            std::vector<int> vn;
+            for (int i : act) {
+                vH.push_back(mols[i].H);
+                vG.push_back(mols[i].G);
+                vF.push_back(mols[i].F);
+                vP.push_back(mols[i].P);
+                vn2.push_back((long long)mols[i].n2);
+                vn.push_back(mols[i].n);
+            }
+            const double *const *tH = (const double *const *)put(vH.data(), cnt * 8);
+            const double *const *tG = (const double *const *)put(vG.data(), cnt * 8);
+            double *const *tF = (double *const *)put(vF.data(), cnt * 8);
+            const double *const *tP = (const double *const *)put(vP.data(), cnt * 8);
+            const long long *tn2 = (const long long *)put(vn2.data(), cnt * 8);
+            const int *tn = (const int *)put(vn.data(), cnt * 4);
+            // error GEMM tables per group: F, P, S, X, W1, M, D, E_slot
+            std::vector<GT> gts;
+            std::vector<const double *> vM, vD;
+            for (Group &g : groups) {
+                std::vector<double *> cols[8];
+                for (int i : g.members) {
+                    if (mols[i].done) continue;
+                    Mol &m = mols[i];
+                    double *v[8] = {m.F, m.P, m.S, m.X, m.W1, m.M, m.D, m.Eh[slot[i]]};
+                    for (int k = 0; k < 8; ++k) cols[k].push_back(v[k]);
+                }
+                if (cols[0].empty()) continue;
+                GT gt{g.n, (int)cols[0].size(), {}, &g};
+                for (int k = 0; k < 8; ++k) gt.t[k] = (double *const *)put(cols[k].data(), cols[k].size() * 8);
+                gts.push_back(gt);
+            }
+            for (int i : act) {
+                vM.push_back(mols[i].M);
+                vD.push_back(mols[i].D);
+            }
+            const double *const *tM = (const double *const *)put(vM.data(), cnt * 8);
+            double *const *tD = (double *const *)put((const void *)vD.data(), cnt * 8);
+            // DIIS matrix row: (E_new . E_s) for every slot s of the molecule
+            std::vector<const double *> va, vb;
+            std::vector<long long> vpn;
+            for (int i : act)
+                for (int s3 : mols[i].slots) {
+                    va.push_back(mols[i].Eh[slot[i]]);
+                    vb.push_back(mols[i].Eh[s3]);
+                    vpn.push_back((long long)mols[i].n2);
+                    pairs.emplace_back(i, s3);
+                }
+            np2 = (int)pairs.size();
+            const double *const *tA = (const double *const *)put(va.data(), np2 * 8);
+            const double *const *tB = (const double *const *)put(vb.data(), np2 * 8);
+            const long long *tpn = (const long long *)put(vpn.data(), np2 * 8);
+            if (overflow) {
+                rc = pkh::fail(PK_ERR_STATE, "pk_scf_rhf_batch: molecule tables exceed their buffer");
+                goto done;
+            }
+            SCF_HIP(hipMemcpyAsync(dstage, hstage.data(), used, hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL(kb_fock, dim3(nblk, cnt), blk, 0, st, tH, tG, tF, tP, tn2, dpart);
+            for (const GT &g : gts)  // W1 = F P, M = W1 S
+                if (rocblas_dgemm_batched(hb, N, N, g.n, g.n, g.n, &one, g.t[0], g.n, g.t[1], g.n, &zero, g.t[4], g.n,
+                                          g.cnt) != rocblas_status_success ||
+                    rocblas_dgemm_batched(hb, N, N, g.n, g.n, g.n, &one, g.t[4], g.n, g.t[2], g.n, &zero, g.t[5], g.n,
+                                          g.cnt) != rocblas_status_success) {
+                    rc = pkh::fail(PK_ERR_HIP, "pk_scf_rhf_batch: batched GEMM failed");
+                    goto done;
+                }
+            hipLaunchKernelGGL(kb_antisym, dim3(nblk, cnt), blk, 0, st, tM, tD, tn);
+            for (const GT &g : gts)  // W1 = X D, E = W1 X
+                if (rocblas_dgemm_batched(hb, N, N, g.n, g.n, g.n, &one, g.t[3], g.n, g.t[6], g.n, &zero, g.t[4], g.n,
+                                          g.cnt) != rocblas_status_success ||
+                    rocblas_dgemm_batched(hb, N, N, g.n, g.n, g.n, &one, g.t[4], g.n, g.t[3], g.n, &zero, g.t[7], g.n,
+                                          g.cnt) != rocblas_status_success) {
+                    rc = pkh::fail(PK_ERR_HIP, "pk_scf_rhf_batch: batched GEMM failed");
+                    goto done;
+                }
+            for (int i : act)
+                SCF_HIP(hipMemcpyAsync(mols[i].Fh[slot[i]], mols[i].F, mols[i].n2 * 8, hipMemcpyDeviceToDevice, st));
+            double *dpart2 = dpart + (size_t)cnt * nblk * 2;
+            hipLaunchKernelGGL(kb_dots, dim3(nblk, np2), blk, 0, st, tA, tB, tpn, dpart2);
+            SCF_HIP(hipGetLastError());
+            SCF_HIP(hipMemcpyAsync(hpart.data(), dpart, ((size_t)cnt + np2) * nblk * 2 * 8, hipMemcpyDeviceToHost, st));
+            SCF_HIP(hipStreamSynchronize(st));
+            }
+            // ---- host: energies, DIIS matrices and coefficients
+            for (int k = 0; k < cnt; ++k) {
+                Mol &m = mols[act[k]];
+                double eph = 0, epf = 0;
+                for (int b2 = 0; b2 < nblk; ++b2) {
+                    eph += hpart[2 * ((size_t)k * nblk + b2)];
+                    epf += hpart[2 * ((size_t)k * nblk + b2) + 1];
+                }
+                m.Etot = 0.5 * (eph + epf) + res[act[k]].nuclear_repulsion;
+                res[act[k]].fock_ms += build_ms;
+            }
+            for (int q = 0; q < np2; ++q) {
+                double v = 0;
+                for (int b2 = 0; b2 < nblk; ++b2) v += hpart[2 * (((size_t)cnt + q) * nblk + b2)];
+                const int i = pairs[q].first, s3 = pairs[q].second;
+                mols[i].Bm[slot[i]][s3] = mols[i].Bm[s3][slot[i]] = v;
+            }
+            // ---- phase B
+            used = 0;
+            std::vector<const double *> vsrc;
+            std::vector<double> vcoef;
+            std::vector<int> vns;
+            std::vector<double *> vout;
+            std::vector<long long> vcn;
+            for (int i : act) {
+                Mol &m = mols[i];
+                if (it == 1) res[i].first_diis_error = std::sqrt(m.Bm[slot[i]][slot[i]] / (double)m.n2);
+                m.Fd = m.F;
+                if (m.slots.size() < 2) continue;
+                const int k = (int)m.slots.size();
+                std::vector<double> A((size_t)(k + 1) * (k + 1), 0.0), rhs(k + 1, 0.0);
+                for (int a2 = 0; a2 < k; ++a2) {
+                    for (int b2 = 0; b2 < k; ++b2) A[(size_t)a2 * (k + 1) + b2] = m.Bm[m.slots[a2]][m.slots[b2]];
+                    A[(size_t)a2 * (k + 1) + k] = A[(size_t)k * (k + 1) + a2] = -1.0;
+                }
+                rhs[k] = -1.0;
+                const std::vector<double> cf = qr_solve(k + 1, A, rhs);
+                for (int a2 = 0; a2 < diis_dim; ++a2) {
+                    vsrc.push_back(a2 < k ? m.Fh[m.slots[a2]] : m.F);
+                    vcoef.push_back(a2 < k ? cf[a2] : 0.0);
+                }
+                vns.push_back(k);
+                vout.push_back(m.D);
+                vcn.push_back((long long)m.n2);
+                m.Fd = m.D;
+            }
+            const int ncomb = (int)vout.size();
+            const double *const *tsrc = (const double *const *)put(vsrc.data(), vsrc.size() * 8);
+            const double *tcoef = (const double *)put(vcoef.data(), vcoef.size() * 8);
+            const int *tns = (const int *)put(vns.data(), vns.size() * 4);
+            double *const *tout = (double *const *)put(vout.data(), vout.size() * 8);
+            const long long *tcn = (const long long *)put(vcn.data(), vcn.size() * 8);
+            std::vector<GT> dts;  // density tables per group: X, Fd, M, W1, C, Pn
+            for (Group &g : groups) {
+                std::vector<double *> cols[6];
+                for (int i : g.members) {
+                    if (mols[i].done) continue;
+                    Mol &m = mols[i];
+                    double *v[6] = {m.X, const_cast<double *>(m.Fd), m.M, m.W1, m.C, m.Pn};
+                    for (int k = 0; k < 6; ++k) cols[k].push_back(v[k]);
+                }
+                if (cols[0].empty()) continue;
+                GT gt{g.n, (int)cols[0].size(), {}, &g};
+                for (int k = 0; k < 6; ++k) gt.t[k] = (double *const *)put(cols[k].data(), cols[k].size() * 8);
+                dts.push_back(gt);
+            }
+            std::vector<const double *> vPn;
+            std::vector<double *> vPo;
+            for (int i : act) {
+                vPn.push_back(mols[i].Pn);
+                vPo.push_back(mols[i].P);
+            }
+            std::vector<long long> vn2b;
+            for (int i : act) vn2b.push_back((long long)mols[i].n2);
+            const double *const *tPn = (const double *const *)put(vPn.data(), cnt * 8);
+            double *const *tPo = (double *const *)put(vPo.data(), cnt * 8);
+            const long long *tn2b = (const long long *)put(vn2b.data(), cnt * 8);
+            if (overflow) {
+                rc = pkh::fail(PK_ERR_STATE, "pk_scf_rhf_batch: molecule tables exceed their buffer");
+                goto done;
+            }
+            SCF_HIP(hipMemcpyAsync(dstage, hstage.data(), used, hipMemcpyHostToDevice, st));
+            if (ncomb > 0)
+                hipLaunchKernelGGL(kb_combine, dim3(nblk, ncomb), blk, 0, st, tsrc, tcoef, tns, diis_dim, tout, tcn);
+            for (const GT &g : dts) {
+                const Group &gr = *g.grp;
+                const int n = g.n, c2 = g.cnt;
+                const rocblas_operation T = rocblas_operation_transpose;
+                if (rocblas_dgemm_batched(hb, N, N, n, n, n, &one, g.t[0], n, g.t[1], n, &zero, g.t[2], n, c2) !=
+                        rocblas_status_success ||
+                    rocblas_dgemm_batched(hb, N, N, n, n, n, &one, g.t[2], n, g.t[0], n, &zero, g.t[3], n, c2) !=
+                        rocblas_status_success ||
+                    rocsolver_dsyevd_batched(hb, rocblas_evect_original, rocblas_fill_upper, n, g.t[3], n, gr.D, n,
+                                             gr.E, n, gr.info, c2) != rocblas_status_success ||
+                    rocblas_dgemm_batched(hb, N, N, n, n, n, &one, g.t[0], n, g.t[3], n, &zero, g.t[4], n, c2) !=
+                        rocblas_status_success ||
+                    rocblas_dgemm_batched(hb, N, T, n, n, gr.nocc, &two, g.t[4], n, g.t[4], n, &zero, g.t[5], n, c2) !=
+                        rocblas_status_success) {
+                    rc = pkh::fail(PK_ERR_HIP, "pk_scf_rhf_batch: batched density step failed");
+                    goto done;
+                }
+            }
+            hipLaunchKernelGGL(kb_diff, dim3(nblk, cnt), blk, 0, st, tPn, tPo, tn2b, dpart);
+            SCF_HIP(hipGetLastError());
+            SCF_HIP(hipMemcpyAsync(hpart.data(), dpart, (size_t)cnt * nblk * 2 * 8, hipMemcpyDeviceToHost, st));
+            SCF_HIP(hipStreamSynchronize(st));
+            for (int k = 0; k < cnt; ++k) {
+                Mol &m = mols[act[k]];
+                pk_scf_result &r = res[act[k]];
+                double dmax = 0, dss = 0;
+                for (int b2 = 0; b2 < nblk; ++b2) {
+                    dmax = std::max(dmax, hpart[2 * ((size_t)k * nblk + b2)]);
+                    dss += hpart[2 * ((size_t)k * nblk + b2) + 1];
+                }
+                const double drms = std::sqrt(dss / (double)m.n2), dE = std::fabs(m.Etot - m.Eprev);
+                m.Eprev = m.Etot;
+                r.iterations = it;
+                r.energy = m.Etot;
+                if (it > 1 && dE < opt->tol_energy && drms < opt->tol_density && dmax < opt->tol_density) {
+                    r.converged = 1;
+                    m.done = true;
+                }
+            }
         }
-        // the new densities are read on the workers' streams; the pointer
-        // tables are rewritten next iteration
-        SCF_HIP(hipStreamSynchronize(st));
-        // (3) per molecule: density change, convergence
-        if ((rc = run([&](int w, int i) {
-                 thread_local std::vector<double> hpart;
-                 hpart.resize(2 * nparts);
-                 return step_b(wst[w], mols[i], res[i], it, hpart);
-             })) != PK_OK)
-            goto done;
-        bool all = true;
-        for (const Mol &m : mols) all = all && m.done;
-        if (all) break;
     }
     for (int i = 0; i < nmol; ++i)
         if (P_out && P_out[i]) SCF_HIP(hipMemcpyAsync(P_out[i], mols[i].P, mols[i].n2 * 8, hipMemcpyDeviceToHost, st));
@@ -1010,10 +1218,6 @@ done:
     {
         const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
         for (int i = 0; i < nmol; ++i) res[i].wall_s = wall;
-    }
-    for (size_t w = 0; w < whb.size(); ++w) {
-        if (whb[w]) rocblas_destroy_handle(whb[w]);
-        if (wst[w]) (void)hipStreamDestroy(wst[w]);
     }
     if (ctx) pk_ctx_destroy(ctx);
     if (hb) rocblas_destroy_handle(hb);
