@@ -101,27 +101,31 @@ def bench_water_clusters(args):
     achieved = dom["model_flops"] / (dom["ms_avg"] * 1e-3) * 1e-12
     print_classes(cls)
     # the same builds one molecule at a time (one context each, eager launches)
-    singles = [FockEngine(s, "os", 1e-12) for s in shells]
-    Ps = eng.unpack(P.cpu().numpy())
-    dPs = [torch.tensor(p, device=dev).contiguous() for p in Ps]
-    dJs = [torch.empty_like(p) for p in dPs]
-    dKs = [torch.empty_like(p) for p in dPs]
-    for e, p, j, k in zip(singles, dPs, dJs, dKs):
-        e.set_stream(stream.cuda_stream)
-        e.jk_device(p.data_ptr(), j.data_ptr(), k.data_ptr())
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    nrep = max(1, args.steps // 4)
-    for _ in range(nrep):
+    seq_ms, dmax = None, None
+    if not args.no_seq:
+        singles = [FockEngine(s, "os", 1e-12) for s in shells]
+        Ps = eng.unpack(P.cpu().numpy())
+        dPs = [torch.tensor(p, device=dev).contiguous() for p in Ps]
+        dJs = [torch.empty_like(p) for p in dPs]
+        dKs = [torch.empty_like(p) for p in dPs]
         for e, p, j, k in zip(singles, dPs, dJs, dKs):
+            e.set_stream(stream.cuda_stream)
             e.jk_device(p.data_ptr(), j.data_ptr(), k.data_ptr())
-    torch.cuda.synchronize()
-    seq_ms = (time.perf_counter() - t0) / nrep * 1e3
-    dmax = max(float((torch.tensor(eng.unpack(J.cpu().numpy())[i]) - dJs[i].cpu()).abs().max()) for i in range(256))
-    print("bench: water clusters nb_total=%d setup %.2fs; batched %.2f ms/build (graph), one-at-a-time %.2f ms; "
-          "max|dJ| batch vs single %.1e" % (eng.nbasis, setup_s, ms, seq_ms, dmax), file=sys.stderr, flush=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        nrep = max(1, args.steps // 4)
+        for _ in range(nrep):
+            for e, p, j, k in zip(singles, dPs, dJs, dKs):
+                e.jk_device(p.data_ptr(), j.data_ptr(), k.data_ptr())
+        torch.cuda.synchronize()
+        seq_ms = (time.perf_counter() - t0) / nrep * 1e3
+        dmax = max(float((torch.tensor(eng.unpack(J.cpu().numpy())[i]) - dJs[i].cpu()).abs().max())
+                   for i in range(256))
+        del singles, dPs, dJs, dKs
+    print("bench: water clusters nb_total=%d setup %.2fs; batched %.2f ms/build (graph), one-at-a-time %s ms; "
+          "max|dJ| batch vs single %s" % (eng.nbasis, setup_s, ms, "%.2f" % seq_ms if seq_ms else "-",
+                                          "%.1e" % dmax if dmax is not None else "-"), file=sys.stderr, flush=True)
     mixed = args.fp32_cutoff is not None
-    del singles, dPs, dJs, dKs
     scf = None
     if not args.no_scf:
         mols_h = [Molecule(Z, X, "6-31g*") for Z, X in mols]
@@ -129,13 +133,13 @@ def bench_water_clusters(args):
         t = time.perf_counter()
         rb = Molecule.rhf_batch(mols_h, **opts)
         t_batch = time.perf_counter() - t
-        t = time.perf_counter()
-        rs = [m.rhf(**opts) for m in mols_h]
-        t_seq = time.perf_counter() - t
-        dE = max(abs(a["energy"] - b["energy"]) for a, b in zip(rb, rs))
-        scf = {"batched_wall_s": round(t_batch, 3), "one_at_a_time_wall_s": round(t_seq, 3),
-               "max_iterations": max(r["iterations"] for r in rb), "converged": all(r["rc"] == 0 for r in rb),
-               "max_abs_dE_batch_vs_single": dE}
+        scf = {"batched_wall_s": round(t_batch, 3), "max_iterations": max(r["iterations"] for r in rb),
+               "converged": all(r["rc"] == 0 for r in rb)}
+        if not args.no_seq:
+            t = time.perf_counter()
+            rs = [m.rhf(**opts) for m in mols_h]
+            scf["one_at_a_time_wall_s"] = round(time.perf_counter() - t, 3)
+            scf["max_abs_dE_batch_vs_single"] = max(abs(a["energy"] - b["energy"]) for a, b in zip(rb, rs))
         if mixed:
             t = time.perf_counter()
             rm = Molecule.rhf_batch(mols_h, fp32_cutoff=args.fp32_cutoff, **opts)
@@ -151,7 +155,7 @@ def bench_water_clusters(args):
                                                                if mixed else ", FP64"),
                    "molecules": 256, "nbasis_total": eng.nbasis, "shell_quartets": st["quartets"],
                    "kernel_launches_per_build": st["kernel_launches"], "one_at_a_time_ms": seq_ms,
-                   "batch_speedup": seq_ms / ms},
+                   "batch_speedup": seq_ms / ms if seq_ms else None},
         "roofline": {"bound": "valu", "kernel": dom_name(dom), "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": None, "kernel_ms": dom["ms_avg"]},
         "cpu_baseline": None,
@@ -193,6 +197,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-scf", action="store_true")
+    ap.add_argument("--no-seq", action="store_true", help="water clusters: skip the one-at-a-time comparisons")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="c20h42-ccpvdz")
