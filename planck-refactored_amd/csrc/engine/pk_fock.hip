@@ -446,32 +446,37 @@ int build_work(pk_ctx *c) {
                 ClassWork w;
                 w.cls = cls;
                 w.prec = prec;
-                std::vector<int> wb, ws, wc, wm, kets;
+                // offsets of every bra's kets and waves (prefix sums), then the
+                // flat arrays filled in parallel
+                std::vector<size_t> koff(tasks.size() + 1, 0), woff(tasks.size() + 1, 0);
                 for (size_t t = 0; t < tasks.size(); ++t) {
-                    const BraOut &o = outs[t];
-                    const size_t n = o.kets.size();
-                    if (n == 0) continue;
-                    const int b = by_cls[tasks[t].first][cb][tasks[t].second];
-                    for (size_t s = 0; s < n; s += group) {
-                        wb.push_back(b);
-                        ws.push_back((int)kets.size() + (int)s);
-                        wc.push_back((int)std::min<size_t>(group, n - s));
-                        wm.push_back(tasks[t].first);
-                    }
-                    kets.insert(kets.end(), o.kets.begin(), o.kets.end());
+                    const size_t n = outs[t].kets.size();
+                    koff[t + 1] = koff[t] + n;
+                    woff[t + 1] = woff[t] + (n + group - 1) / group;
                     w.nquart += (int64_t)n;
-                    w.nprimq += o.nprimq;
-                    w.flops += o.flops;
+                    w.nprimq += outs[t].nprimq;
+                    w.flops += outs[t].flops;
                 }
+                if (koff.back() > (size_t)INT32_MAX) return fail(PK_ERR_ARG, "build_work: more than 2^31 quartets in one class");
+                const size_t nw = woff.back();
+                std::vector<int> kets(koff.back()), packed(4 * nw);
+                parallel_for(tasks.size(), [&](size_t t) {
+                    BraOut &o = outs[t];
+                    const size_t n = o.kets.size();
+                    if (n == 0) return;
+                    std::copy(o.kets.begin(), o.kets.end(), kets.begin() + koff[t]);
+                    const int b = by_cls[tasks[t].first][cb][tasks[t].second];
+                    size_t wi = woff[t];
+                    for (size_t s0 = 0; s0 < n; s0 += group, ++wi) {
+                        packed[wi] = b;
+                        packed[nw + wi] = (int)(koff[t] + s0);
+                        packed[2 * nw + wi] = (int)std::min<size_t>(group, n - s0);
+                        packed[3 * nw + wi] = tasks[t].first;
+                    }
+                    std::vector<int>().swap(o.kets);
+                });
                 if (w.nquart == 0) continue;
-                if (kets.size() > (size_t)INT32_MAX) return fail(PK_ERR_ARG, "build_work: more than 2^31 quartets in one class");
-                w.nwaves = (int)wb.size();
-                std::vector<int> packed;
-                packed.reserve(4 * wb.size());
-                packed.insert(packed.end(), wb.begin(), wb.end());
-                packed.insert(packed.end(), ws.begin(), ws.end());
-                packed.insert(packed.end(), wc.begin(), wc.end());
-                packed.insert(packed.end(), wm.begin(), wm.end());
+                w.nwaves = (int)nw;
                 int rc;
                 if ((rc = dev_upload(&w.d_wave, packed, c->stream)) != PK_OK) return rc;
                 if ((rc = dev_upload(&w.d_kets, kets, c->stream)) != PK_OK) return rc;
