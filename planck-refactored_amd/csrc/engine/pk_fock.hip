@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <numeric>
 #include <string>
 #include <thread>
@@ -179,12 +180,17 @@ struct ClassWork {
 };
 
 template <typename T>
-int dev_upload(T **dst, const std::vector<T> &src, hipStream_t s) {
+int dev_upload(T **dst, const T *src, size_t n, hipStream_t s) {
     *dst = nullptr;
-    if (src.empty()) return PK_OK;
-    PK_HIP(hipMalloc((void **)dst, src.size() * sizeof(T)));
-    PK_HIP(hipMemcpyAsync(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    if (n == 0) return PK_OK;
+    PK_HIP(hipMalloc((void **)dst, n * sizeof(T)));
+    PK_HIP(hipMemcpyAsync(*dst, src, n * sizeof(T), hipMemcpyHostToDevice, s));
     return PK_OK;
+}
+
+template <typename T>
+int dev_upload(T **dst, const std::vector<T> &src, hipStream_t s) {
+    return dev_upload(dst, src.data(), src.size(), s);
 }
 
 }  // namespace
@@ -459,12 +465,15 @@ int build_work(pk_ctx *c) {
                 }
                 if (koff.back() > (size_t)INT32_MAX) return fail(PK_ERR_ARG, "build_work: more than 2^31 quartets in one class");
                 const size_t nw = woff.back();
-                std::vector<int> kets(koff.back()), packed(4 * nw);
+                // the ket array is written entirely below: no value-initialisation
+                // (it holds one index per quartet, up to GBs for large batches)
+                std::unique_ptr<int[]> kets(new int[std::max<size_t>(1, koff.back())]);
+                std::vector<int> packed(4 * nw);
                 parallel_for(tasks.size(), [&](size_t t) {
                     BraOut &o = outs[t];
                     const size_t n = o.kets.size();
                     if (n == 0) return;
-                    std::copy(o.kets.begin(), o.kets.end(), kets.begin() + koff[t]);
+                    std::copy(o.kets.begin(), o.kets.end(), kets.get() + koff[t]);
                     const int b = by_cls[tasks[t].first][cb][tasks[t].second];
                     size_t wi = woff[t];
                     for (size_t s0 = 0; s0 < n; s0 += group, ++wi) {
@@ -479,7 +488,7 @@ int build_work(pk_ctx *c) {
                 w.nwaves = (int)nw;
                 int rc;
                 if ((rc = dev_upload(&w.d_wave, packed, c->stream)) != PK_OK) return rc;
-                if ((rc = dev_upload(&w.d_kets, kets, c->stream)) != PK_OK) return rc;
+                if ((rc = dev_upload(&w.d_kets, kets.get(), koff.back(), c->stream)) != PK_OK) return rc;
                 c->st.quartets += w.nquart;
                 c->st.primitive_quartets += w.nprimq;
                 c->st.model_flops += w.flops;
