@@ -1,19 +1,30 @@
 #!/usr/bin/env python3
-"""bench.py — J/K Fock builds per second of the MI355X engine.
+"""bench.py — J/K Fock builds per second of the MI355X engine, plain FP64 and
+mixed precision, gated by the SCF energy error of the mixed runs.
 
-Workload (one GPU): RHF two-electron Fock build J[P], K[P] of all-trans
-n-C20H42 (idealised geometry: C-C 1.54 A, C-H 1.09 A, tetrahedral angles) in
-the Cartesian cc-pVDZ basis (510 functions, up to (dd|dd) quartets), Schwarz
-threshold 1e-12, FP64 throughout.  A step is one build from a device-resident
-density (seeded random symmetric matrix; the FP64 path has no density-dependent
-screening, so the work does not depend on its values) to device-resident J and
-K.  After the timed steps the full RHF SCF of the same molecule runs once
-(core guess, DIIS 8, tol_energy = tol_density = 1e-10) for the SCF wall time.
+Default workload (one GPU): the RHF two-electron Fock build J[P], K[P] of
+all-trans n-C10H22 (idealised geometry: C-C 1.54 A, C-H 1.09 A, tetrahedral
+angles) in the Cartesian cc-pVTZ basis (680 functions, f shells, so every
+quartet class up to (ff|ff) runs), Schwarz threshold 1e-12.  A step is one
+build from a device-resident density (seeded random symmetric matrix) to
+device-resident J and K.
 
-Printed: one JSON line with the throughput, the roofline of the dominant ERI
-kernel (flop model v1 over its measured HIP-event time) and the CPU baseline
-(the oracle's reference-faithful Fock build, a bounded sample of bra AO pairs
-on the host cores, scaled to one full build).
+Measured, in this order:
+  1. the plain FP64 build: W untimed + K timed steps;
+  2. the mixed-precision build (quartets whose Schwarz bound lies below the
+     FP32 cutoff computed in FP32, everything else FP64) at each cutoff of
+     --cutoffs (default 1e-4 .. 1e-7), W + K steps each;
+  3. the RHF SCF (core guess, DIIS 8) in FP64 and at each cutoff: wall time,
+     iterations and |dE| against the FP64 SCF;
+  4. the CPU baseline: the oracle's reference-faithful Fock build on a bounded
+     sample of bra AO pairs on the host cores, scaled to one build.
+The headline `value` is the fastest build whose SCF energy lies within 1e-6 Eh
+of the plain FP64 GPU energy (the plain build itself qualifies).  The roofline
+is that configuration's dominant ERI class: flop model v1 over its HIP-event
+time in the timed steps.
+
+Other workloads: c20h42-ccpvdz (510 functions, FP64 and mixed) and
+water-clusters (256 seeded clusters in one batched, graph-replayed build).
 """
 import argparse
 import json
@@ -163,6 +174,20 @@ def bench_water_clusters(args):
     print(json.dumps(out), flush=True)
 
 
+def cpu_info():
+    """CPU model of this host (lscpu / /proc/cpuinfo) and its logical CPUs."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
 def cpu_baseline(Z, X, basis, seconds, threads):
     """Oracle Fock build (per-AO-function Obara-Saika, the reference's
     algorithm) on every stride-th bra AO pair, scaled by the stride."""
@@ -177,8 +202,8 @@ def cpu_baseline(Z, X, basis, seconds, threads):
     A = rng.normal(size=(n, n)) * 0.05
     P = A + A.T
     ref.jk_sample(P, stride=npairs, offset=0, nthreads=threads)  # Schwarz table outside the timing
-    stride = 16384
-    for _ in range(3):
+    stride = 65536
+    for _ in range(4):
         t = time.perf_counter()
         _, _, nq = ref.jk_sample(P, stride=stride, offset=1, nthreads=threads)
         dt = time.perf_counter() - t
@@ -186,44 +211,39 @@ def cpu_baseline(Z, X, basis, seconds, threads):
             break
         stride = max(64, int(stride * max(dt, 1e-3) / seconds))
     full = dt * stride
+    model, ncpu = cpu_info()
     return {"value": 1.0 / full, "unit": "builds/s", "cores": threads, "kind": "port",
-            "sample": "oracle J/K over every %d-th of %d bra AO pairs (%.0f AO quartets, %.1f s), "
-                      "scaled x%d to one build" % (stride, npairs, nq, dt, stride)}
+            "cpu_model": model, "host_logical_cpus": ncpu,
+            "sample": "oracle J/K (reference-faithful per-function OS, OpenMP) over every %d-th of %d bra AO pairs "
+                      "(%.0f AO quartets, %.1f s on %d threads), scaled x%d to one build" % (
+                          stride, npairs, nq, dt, threads, stride)}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--no-scf", action="store_true")
-    ap.add_argument("--no-seq", action="store_true", help="water clusters: skip the one-at-a-time comparisons")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="c20h42-ccpvdz")
-    ap.add_argument("--copy-calibration", action="store_true",
-                    help="only copy 512 MiB device to device 3 times (scales rocprofv3 FETCH_SIZE / WRITE_SIZE)")
-    ap.add_argument("--fp32-cutoff", type=float, default=None,
-                    help="mixed precision: quartets with Schwarz bound below this run in FP32")
-    args = ap.parse_args()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this engine measures one GPU; multi-GPU runs are not part of this build "
-              "(the J/K build shards by quartet task list, see DESIGN.md)", file=sys.stderr)
-        sys.exit(2)
-
-    if args.workload == "water-clusters":
-        return bench_water_clusters(args)
+def timed_builds(eng, P, J, K, steps, warmup):
+    """W untimed builds, then K timed ones (per-class HIP events on the
+    engine's stream); returns (ms per build, class stats with ms_avg)."""
     import torch
-    if args.copy_calibration:
-        n = 64 * 1024 * 1024
-        a = torch.ones(n, dtype=torch.float64, device="cuda")
-        b = torch.empty_like(a)
-        torch.cuda.synchronize()
-        for _ in range(3):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes": n * 8, "copies": 3}), flush=True)
-        return
+    for _ in range(max(warmup, 1)):
+        eng.jk_device(P.data_ptr(), J.data_ptr(), K.data_ptr())
+    torch.cuda.synchronize()
+    eng.profile(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eng.jk_device(P.data_ptr(), J.data_ptr(), K.data_ptr())
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    eng.profile(False)
+    cls = eng.class_stats()
+    for c in cls:
+        c["ms_avg"] = c["ms_total"] / max(c["builds"], 1)
+    return elapsed / steps * 1e3, elapsed, cls
+
+
+def bench_alkane(args):
+    import numpy as np
+    import torch
+    from pkfock.engine import PREC_MIXED
     from pkfock.host import Molecule
     from pkfock.molecules import alkane
 
@@ -233,56 +253,41 @@ def main():
     mol = Molecule(Z, X, basis)
     t = time.perf_counter()
     eng = mol.engine("os", tol_eri=1e-12)
-    mixed = args.fp32_cutoff is not None
-    if mixed:
-        from pkfock.engine import PREC_MIXED
-        eng.set("os", 1e-12, precision=PREC_MIXED, fp32_cutoff=args.fp32_cutoff)
-    setup_s = time.perf_counter() - t
     n = eng.nbasis
     g = torch.Generator().manual_seed(0)
     A = torch.randn(n, n, generator=g, dtype=torch.float64) * 0.05
     P = (A + A.T).contiguous().to(dev)
-    J = torch.empty_like(P)
-    K = torch.empty_like(P)
+    J, K = torch.empty_like(P), torch.empty_like(P)
     stream = torch.cuda.current_stream(dev)
     eng.set_stream(stream.cuda_stream)
-    t = time.perf_counter()
-    for _ in range(max(args.warmup, 1)):
-        eng.jk_device(P.data_ptr(), J.data_ptr(), K.data_ptr())
+    eng.jk_device(P.data_ptr(), J.data_ptr(), K.data_ptr())  # work lists
     torch.cuda.synchronize()
-    warm_s = time.perf_counter() - t
-    print("bench: %s nb=%d setup %.2fs warmup %.2fs" % (args.workload, n, setup_s, warm_s), file=sys.stderr,
-          flush=True)
+    setup_s = time.perf_counter() - t
+    print("bench: %s nb=%d setup %.2fs" % (args.workload, n, setup_s), file=sys.stderr, flush=True)
 
-    eng.profile(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        eng.jk_device(P.data_ptr(), J.data_ptr(), K.data_ptr())
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    eng.profile(False)
-    elapsed = t1 - t0
-    ms = elapsed / args.steps * 1e3
-    st = eng.stats()
-    cls = eng.class_stats()
-    for c in cls:
-        c["ms_avg"] = c["ms_total"] / max(c["builds"], 1)
-    dom = max(cls, key=lambda c: c["ms_avg"])
-    achieved = dom["model_flops"] / (dom["ms_avg"] * 1e-3) * 1e-12
-    print("bench: %d steps %.2f ms/step; model %.3e flop/build = %.2f TFLOP/s whole build" % (
-        args.steps, ms, st["model_flops"], st["model_flops"] / ms * 1e-9), file=sys.stderr, flush=True)
-    print_classes(cls)
+    cutoffs = [float(c) for c in args.cutoffs.split(",")] if args.cutoffs else []
+    runs = []  # one entry per precision configuration
+    for cut in [None] + cutoffs:
+        if cut is None:
+            eng.set("os", 1e-12)
+        else:
+            eng.set("os", 1e-12, precision=PREC_MIXED, fp32_cutoff=cut)
+        ms, elapsed, cls = timed_builds(eng, P, J, K, args.steps, args.warmup)
+        st = eng.stats()
+        dom = max(cls, key=lambda c: c["ms_avg"])
+        n32 = sum(c["quartets"] for c in cls if c["precision"])
+        run = {"cutoff": cut, "ms": ms, "builds_per_s": args.steps / elapsed, "elapsed": elapsed, "cls": cls,
+               "dom": dom, "stats": st, "fp32_quartet_frac": n32 / max(st["quartets"], 1)}
+        runs.append(run)
+        print("bench: %s %d steps %.2f ms/build (%.2f builds/s); model %.3e flop/build = %.2f TFLOP/s; "
+              "FP32 quartets %.1f%%" % ("FP64" if cut is None else "mixed %g" % cut, args.steps, ms,
+                                        run["builds_per_s"], st["model_flops"], st["model_flops"] / ms * 1e-9,
+                                        100 * run["fp32_quartet_frac"]), file=sys.stderr, flush=True)
+        if args.verbose or cut is None:
+            print_classes(cls)
+    eng.set("os", 1e-12)
 
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.workload.replace("-", "_"))
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            tr = json.load(f)
-        traffic = tr.get(dom_name(dom))
-
-    # one-electron integrals (device kernel; the call includes the copy of
-    # S, T, V to the host) and the UHF variant (two builds, device arrays)
+    # UHF variant (one pass over the ERIs) and one-electron integrals
     t = time.perf_counter()
     mol.one_electron()
     one_ms = (time.perf_counter() - t) * 1e3
@@ -296,15 +301,32 @@ def main():
         eng.jk_uhf_device(P.data_ptr(), Pb.data_ptr(), Jt.data_ptr(), Ka.data_ptr(), Kb.data_ptr())
     torch.cuda.synchronize()
     uhf_ms = (time.perf_counter() - t) / nu * 1e3
-    print("bench: one-electron S, T, V %.2f ms (incl. copy to host); UHF J/K %.2f ms per call" % (one_ms, uhf_ms),
-          file=sys.stderr, flush=True)
+    print("bench: one-electron S, T, V %.2f ms (incl. copy to host); UHF J/K %.2f ms per call (RHF %.2f)" % (
+        one_ms, uhf_ms, runs[0]["ms"]), file=sys.stderr, flush=True)
 
-    scf = None
+    # SCF per configuration: |dE| against the plain FP64 GPU SCF
     if not args.no_scf:
-        r = mol.rhf(engine="os", tol_eri=1e-12, fp32_cutoff=args.fp32_cutoff)
-        scf = {"wall_s": round(r["wall_s"], 3), "iterations": r["iterations"], "converged": bool(r["converged"]),
-               "energy": r["energy"], "fock_ms_total": round(r["fock_ms"], 1)}
-        print("bench: SCF %s" % scf, file=sys.stderr, flush=True)
+        scf_kw = dict(engine="os", tol_eri=1e-12, tol_energy=args.scf_tol_energy, tol_density=args.scf_tol_density)
+        for run in runs:
+            r = mol.rhf(fp32_cutoff=run["cutoff"], **scf_kw)
+            run["scf"] = {"wall_s": round(r["wall_s"], 3), "iterations": r["iterations"],
+                          "converged": bool(r["converged"]), "energy": r["energy"],
+                          "fock_ms_total": round(r["fock_ms"], 1)}
+            run["scf"]["abs_dE_vs_fp64"] = abs(r["energy"] - runs[0]["scf"]["energy"])
+            print("bench: SCF %s %s" % ("FP64" if run["cutoff"] is None else "mixed %g" % run["cutoff"], run["scf"]),
+                  file=sys.stderr, flush=True)
+
+    # headline: the fastest build whose SCF energy is within 1e-6 Eh of FP64
+    ok = [r for r in runs if r["cutoff"] is None or
+          (r.get("scf") and r["scf"]["converged"] and r["scf"]["abs_dE_vs_fp64"] <= 1e-6)]
+    best = max(ok, key=lambda r: r["builds_per_s"])
+    dom = best["dom"]
+    achieved = dom["model_flops"] / (dom["ms_avg"] * 1e-3) * 1e-12
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.workload.replace("-", "_"))
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get(dom_name(dom))
 
     cpu = None
     if not args.no_cpu:
@@ -314,24 +336,84 @@ def main():
         except Exception as e:  # the baseline is reported, never required
             print("bench: cpu baseline unavailable: %s" % e, file=sys.stderr)
 
+    mixed = best["cutoff"] is not None
     peak = PEAK_FP32_TFLOPS if dom["precision"] else PEAK_FP64_TFLOPS
+    sweep = []
+    for r in runs:
+        e = {"precision": "fp64" if r["cutoff"] is None else "mixed", "fp32_cutoff": r["cutoff"],
+             "ms_per_build": round(r["ms"], 3), "builds_per_s": round(r["builds_per_s"], 4),
+             "fp32_quartet_frac": round(r["fp32_quartet_frac"], 4),
+             "speedup_vs_fp64": round(runs[0]["ms"] / r["ms"], 4)}
+        if r.get("scf"):
+            e.update({"scf_wall_s": r["scf"]["wall_s"], "scf_iterations": r["scf"]["iterations"],
+                      "scf_energy": r["scf"]["energy"], "abs_dE_vs_fp64": r["scf"]["abs_dE_vs_fp64"],
+                      "scf_converged": r["scf"]["converged"]})
+        sweep.append(e)
+    st = best["stats"]
     out = {
-        "metric": METRIC, "value": args.steps / elapsed, "unit": "builds/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "strong",
+        "metric": METRIC, "value": best["builds_per_s"], "unit": "builds/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": best["ms"], "higher_is_better": True, "scaling": "strong",
         "vs_baseline": None, "dtype": "f64+f32" if mixed else "f64", "data": "synthetic",
-        "config": {"workload": desc + (", mixed FP32 below Schwarz %g" % args.fp32_cutoff if mixed else ", FP64"),
-                   "nbasis": n, "shell_quartets": st["quartets"], "primitive_quartets": st["primitive_quartets"],
+        "config": {"workload": desc + (", mixed: FP32 below Schwarz %g" % best["cutoff"] if mixed else ", FP64"),
+                   "nbasis": n, "precision": "mixed" if mixed else "fp64", "fp32_cutoff": best["cutoff"],
+                   "shell_quartets": st["quartets"], "primitive_quartets": st["primitive_quartets"],
                    "model_flops_per_build": st["model_flops"],
-                   "whole_build_tflops": st["model_flops"] / ms * 1e-9},
+                   "whole_build_tflops": st["model_flops"] / best["ms"] * 1e-9,
+                   "plain_fp64_builds_per_s": runs[0]["builds_per_s"],
+                   "speedup_vs_plain_fp64": runs[0]["ms"] / best["ms"],
+                   "scf_wall_s": best["scf"]["wall_s"] if best.get("scf") else None,
+                   "abs_dE_vs_fp64": best["scf"]["abs_dE_vs_fp64"] if best.get("scf") else None,
+                   "scf_tol_energy": args.scf_tol_energy, "scf_tol_density": args.scf_tol_density},
+        "sweep": sweep,
         "roofline": {"bound": "valu", "kernel": dom_name(dom), "achieved": achieved, "peak": peak,
                      "unit": "TFLOP/s", "frac": achieved / peak, "traffic": traffic,
                      "kernel_ms": dom["ms_avg"], "kernel_model_flops": dom["model_flops"]},
         "cpu_baseline": cpu,
-        "scf": scf,
         "one_electron_ms": round(one_ms, 2),
         "uhf_ms_per_build": round(uhf_ms, 2),
     }
     print(json.dumps(out), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-scf", action="store_true")
+    ap.add_argument("--no-seq", action="store_true", help="water clusters: skip the one-at-a-time comparisons")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--verbose", action="store_true", help="per-class times of every configuration")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="c10h22-ccpvtz")
+    ap.add_argument("--cutoffs", default="1e-4,1e-5,1e-6,1e-7",
+                    help="FP32 Schwarz cutoffs of the mixed-precision sweep ('' for FP64 only)")
+    ap.add_argument("--scf-tol-energy", type=float, default=1e-9)
+    ap.add_argument("--scf-tol-density", type=float, default=1e-7)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 512 MiB device to device 3 times (scales rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--fp32-cutoff", type=float, default=None,
+                    help="water clusters, mixed precision: quartets with Schwarz bound below this run in FP32")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this engine measures one GPU; multi-GPU runs are not part of this build "
+              "(the J/K build would shard by quartet task list over GPUs)", file=sys.stderr)
+        sys.exit(2)
+
+    if args.copy_calibration:
+        import torch
+        n = 64 * 1024 * 1024
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n * 8, "copies": 3}), flush=True)
+        return
+    if args.workload == "water-clusters":
+        return bench_water_clusters(args)
+    return bench_alkane(args)
 
 
 def dom_name(c):

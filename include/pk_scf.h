@@ -83,7 +83,9 @@ int pk_scf_rhf(const pk_basis *b, const pk_scf_options *opt, pk_scf_result *res,
  * pk_scf_rhf): the Fock builds of one iteration are one batched, hipGraph-
  * replayed build of every molecule (pk_batch_create); the per-molecule algebra
  * follows on the same stream.  res[nmol]; P_out[i] and guess[i] (host nb_i^2
- * densities) may be NULL, as may the arrays.  wall_s is the batch's wall time;
+ * densities) may be NULL, as may the arrays.  guess_fallback applies per
+ * molecule as in pk_scf_rhf (the first DIIS errors of the supplied and the
+ * core densities come from two batched builds).  wall_s is the batch's wall time;
  * fock_ms the summed batched-build time over the molecule's iterations.  Not
  * in the reference (its SCF handles one molecule). */
 int pk_scf_rhf_batch(const pk_basis *const *bases, int nmol, const pk_scf_options *opt, pk_scf_result *res,

@@ -3,8 +3,9 @@
 numpy restatement of the network of include/pk_mlp.h: n_in -> hidden -> hidden
 -> n_out with tanh, parameters packed W1 [n_in][hidden], b1, W2, b2, W3
 [hidden][n_out], b3; loss L = mean over rows and outputs of (y - t)^2; Adam with
-bias correction.  Computed in float64.  Pinned in tests/test_mlp_oracle.py by
-central finite differences of the loss.  No counterpart in the reference.
+bias correction.  Computed in float64.  Pinned in tests/test_guess_cpu.py
+(test_mlp_oracle_gradient_by_finite_differences) by central finite
+differences of the loss.  No counterpart in the reference.
 """
 import numpy as np
 

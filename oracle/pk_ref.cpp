@@ -913,9 +913,19 @@ bool auto_prefers_rys(const int la[3], const int lb[3], const int lc[3], const i
 
 enum { ENG_OS = 0, ENG_RYS = 1, ENG_AUTO = 2 };
 
+constexpr int kRysMaxOracleRoots = 5;  // rys_rule is pinned by moments up to n = 5
+
+int total_l(const int *a, const int *b, const int *c, const int *d) {
+    return a[0] + a[1] + a[2] + b[0] + b[1] + b[2] + c[0] + c[1] + c[2] + d[0] + d[1] + d[2];
+}
+
 double contracted_eri(const pkref_ctx &c, int eng, const Pair &ab, const Pair &cd) {
     const Func &A = c.funcs[ab.i], &B = c.funcs[ab.j], &C = c.funcs[cd.i], &D = c.funcs[cd.j];
     bool rys = eng == ENG_RYS || (eng == ENG_AUTO && auto_prefers_rys(A.l, B.l, C.l, D.l));
+    // The reference's moment/Stieltjes/Jacobi rule loses accuracy from n = 6
+    // roots (root errors 9e-9 at n = 7, 6e-3 at n = 9): the oracle does not
+    // use it there and computes those components by Obara-Saika instead.
+    if (rys && total_l(A.l, B.l, C.l, D.l) / 2 + 1 > kRysMaxOracleRoots) rys = false;
     double v = 0.0;
     for (const PrimPair &p : ab.pp)
         for (const PrimPair &q : cd.pp)
@@ -1307,6 +1317,52 @@ int pkref_fock_jk(pkref_ctx *c, const double *P, double *J, double *K, int nthre
     return pkref_fock_jk_sample(c, P, J, K, nthreads, 1, 0, nullptr);
 }
 
+// Single elements of J and K, each from its own sums over every AO quartet
+// that passes the same screening test as the full build (Q_ij Q_kl is
+// invariant under the 8 permutations, so each term below is kept exactly
+// when its unique quartet is):
+//   J_ij = sum_{k,l} (ij|kl) P_kl,   K_ij = sum_{k,l} (ik|jl) P_kl.
+// Equal to the corresponding elements of pkref_fock_jk (os.cpp:1509-1522),
+// at a cost of O(nb^2) ERIs per element instead of the whole build, so large
+// molecules can be checked on a sample of elements.  Summed serially in long
+// double per element: the result does not depend on the thread count.
+int pkref_jk_elements(pkref_ctx *c, const double *P, int nel, const int *ei, const int *ej, double *J, double *K,
+                      int nthreads) {
+    ensure_schwarz(*c);
+    const int nb = c->nb;
+    for (int e = 0; e < nel; ++e)
+        if (ei[e] < 0 || ei[e] >= nb || ej[e] < 0 || ej[e] >= nb) {
+            g_err = "pkref_jk_elements: element index out of range";
+            return -1;
+        }
+    const int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#pragma omp parallel for num_threads(nt) schedule(dynamic)
+    for (int e = 0; e < nel; ++e) {
+        const int i = ei[e], j = ej[e];
+        const Pair &ij = c->pairs[pair_index(nb, i, j)];
+        const double qij = c->Q[(size_t)i * nb + j];
+        long double sj = 0.0L, sk = 0.0L;
+        for (int k = 0; k < nb; ++k)
+            for (int l = k; l < nb; ++l) {
+                if (qij * c->Q[(size_t)k * nb + l] < c->tol) continue;
+                const double v = contracted_eri(*c, c->engine, ij, c->pairs[pair_index(nb, k, l)]);
+                sj += (long double)v * (k == l ? P[(size_t)k * nb + l] : P[(size_t)k * nb + l] + P[(size_t)l * nb + k]);
+            }
+        for (int k = 0; k < nb; ++k) {
+            const Pair &ik = c->pairs[pair_index(nb, i, k)];
+            const double qik = c->Q[(size_t)i * nb + k];
+            for (int l = 0; l < nb; ++l) {
+                if (qik * c->Q[(size_t)j * nb + l] < c->tol) continue;
+                const double v = contracted_eri(*c, c->engine, ik, c->pairs[pair_index(nb, j, l)]);
+                sk += (long double)v * P[(size_t)k * nb + l];
+            }
+        }
+        J[e] = (double)sj;
+        K[e] = (double)sk;
+    }
+    return 0;
+}
+
 double pkref_boys(int n, double x) {
     init_boys_table();
     return boys(n, x);
@@ -1314,7 +1370,17 @@ double pkref_boys(int n, double x) {
 
 void pkref_set_boys_exact(int on) { g_boys_exact = on ? 1 : 0; }
 
-void pkref_rys_rule(int n, double T, double *roots, double *weights) { rys_rule(n, T, roots, weights); }
+int pkref_rys_rule(int n, double T, double *roots, double *weights) {
+    // refused beyond the roots its moment test pins (see contracted_eri)
+    if (n < 1 || n > kRysMaxOracleRoots) {
+        g_err = "pkref_rys_rule: n = " + std::to_string(n) + " is outside 1.." + std::to_string(kRysMaxOracleRoots) +
+                " (the moment method is inaccurate beyond)";
+        for (int i = 0; i < n && i < 11; ++i) roots[i] = weights[i] = std::nan("");
+        return -1;
+    }
+    rys_rule(n, T, roots, weights);
+    return 0;
+}
 
 int pkref_auto_prefers_rys(const int *lA, const int *lB, const int *lC, const int *lD) {
     return auto_prefers_rys(lA, lB, lC, lD) ? 1 : 0;

@@ -49,6 +49,13 @@ int pkref_fock_jk(pkref_ctx *ctx, const double *P, double *J, double *K, int nth
 int pkref_fock_jk_sample(pkref_ctx *ctx, const double *P, double *J, double *K,
                          int nthreads, long stride, long offset, double *n_quartets);
 
+/* Elements (ei[e], ej[e]) of J and K of density P, e < nel, each summed on
+ * its own over every AO quartet the full build keeps (same screening): equal
+ * to those elements of pkref_fock_jk at O(nb^2) ERIs per element, so large
+ * molecules can be checked on a sample of elements. */
+int pkref_jk_elements(pkref_ctx *ctx, const double *P, int nel, const int *ei, const int *ej, double *J,
+                      double *K, int nthreads);
+
 /* RHF SCF exactly as run_rhf (core guess).  hist gets E_total per iteration. */
 int pkref_rhf(pkref_ctx *ctx, int charge, double tol_energy, double tol_density,
               int diis_dim, int max_cycles, double *energy, int *iterations,
@@ -60,8 +67,10 @@ double pkref_boys(int n, double x);
  * attraction) uses the accurate series instead of the reference's lookup;
  * 0: the lookup (default).  Process-wide; not the reference's behaviour. */
 void pkref_set_boys_exact(int on);
-/* n-point Rys rule by the moment/Stieltjes/Jacobi construction */
-void pkref_rys_rule(int n, double T, double *roots, double *weights);
+/* n-point Rys rule by the moment/Stieltjes/Jacobi construction, 1 <= n <= 5
+ * (returns -1 and NaNs beyond: the method is inaccurate there; the oracle's
+ * Rys and auto engines compute components needing more roots by OS) */
+int pkref_rys_rule(int n, double T, double *roots, double *weights);
 /* auto-engine routing decision for one Cartesian component quartet */
 int pkref_auto_prefers_rys(const int *lA, const int *lB, const int *lC, const int *lD);
 

@@ -966,6 +966,9 @@ int pk_ctx_set(pk_ctx *c, int engine, double tol, int precision, double cutoff) 
     if (precision == PK_PREC_MIXED && !(cutoff >= 0.0))
         return fail(PK_ERR_ARG, "pk_ctx_set: the FP32 cutoff must be >= 0");
     if (precision == PK_PREC_FP64) cutoff = 0.0;
+    // the kernel of each class (OS or Rys) is chosen at launch time, so a
+    // captured graph holds the previous engine's kernels
+    if (engine != c->engine) drop_graph(c);
     c->engine = engine;
     if (tol != c->tol || precision != c->precision || cutoff != c->cutoff) {
         c->tol = tol;

@@ -796,6 +796,7 @@ int pk_scf_rhf_batch(const pk_basis *const *bases, int nmol, const pk_scf_option
     std::vector<int64_t> lay_off(nmol);
     size_t ntot = 0;
     double *Pp = nullptr, *Gp = nullptr;
+    bool have_G = false;  // Gp already holds G[Pp] (guess fallback)
     (void)one;
     auto gemm_h = [&](rocblas_handle h, Mol &m, rocblas_operation ta, rocblas_operation tb, int k, const double *A,
                       const double *B, double *Cm) {
@@ -923,6 +924,66 @@ int pk_scf_rhf_batch(const pk_basis *const *bases, int nmol, const pk_scf_option
             }
         }
     }
+    // Guess fallback (the rule of pk_scf_rhf): for every molecule given a
+    // guess density, the first DIIS error of that density and of the core
+    // guess, each from one batched build of all molecules; the smaller one
+    // starts the SCF.  The builds of the chosen densities are kept, so the
+    // first iteration does not build again.
+    if (opt->guess_fallback && guess) {
+        std::vector<int> gm;
+        for (int i = 0; i < nmol; ++i)
+            if (guess[i]) gm.push_back(i);
+        if (!gm.empty()) {
+            double *Pc = dev.alloc(ntot), *Gc = dev.alloc(ntot);
+            if (!Pc || !Gc) {
+                rc = pkh::fail(PK_ERR_ALLOC, "pk_scf_rhf_batch: device allocation failed");
+                goto done;
+            }
+            // RMS of X (F P S - S P F) X with F = H + G for molecule m
+            auto first_error = [&](Mol &m, const double *Pm, const double *Gm, double &err) -> int {
+                const int n = m.n, g2 = (int)((m.n2 + 255) / 256);
+                hipLaunchKernelGGL(k_add, dim3(g2), dim3(256), 0, st, m.H, Gm, m.F, m.n2);
+                if (gemm(m, rocblas_operation_none, rocblas_operation_none, n, m.F, Pm, m.W1) != rocblas_status_success ||
+                    gemm(m, rocblas_operation_none, rocblas_operation_none, n, m.W1, m.S, m.M) != rocblas_status_success)
+                    return pkh::fail(PK_ERR_HIP, "pk_scf_rhf_batch: gemm failed");
+                hipLaunchKernelGGL(k_antisym, dim3(g2), dim3(256), 0, st, m.M, m.D, n);
+                if (gemm(m, rocblas_operation_none, rocblas_operation_none, n, m.X, m.D, m.W1) != rocblas_status_success ||
+                    gemm(m, rocblas_operation_none, rocblas_operation_none, n, m.W1, m.X, m.M) != rocblas_status_success)
+                    return pkh::fail(PK_ERR_HIP, "pk_scf_rhf_batch: gemm failed");
+                double v = 0;
+                if (rocblas_set_pointer_mode(hb, rocblas_pointer_mode_host) != rocblas_status_success ||
+                    rocblas_ddot(hb, (rocblas_int)m.n2, m.M, 1, m.M, 1, &v) != rocblas_status_success)
+                    return pkh::fail(PK_ERR_HIP, "pk_scf_rhf_batch: ddot failed");
+                err = std::sqrt(v / (double)m.n2);
+                return PK_OK;
+            };
+            SCF_HIP(hipMemcpyAsync(Pc, Pp, ntot * 8, hipMemcpyDeviceToDevice, st));
+            for (int i : gm)
+                if ((rc = density(mols[i], mols[i].H, Pc + lay_off[i])) != PK_OK) goto done;
+            std::vector<double> eg(nmol, 0.0), ec(nmol, 0.0);
+            for (int pass = 0; pass < 2; ++pass) {
+                double *Pb = pass == 0 ? Pp : Pc, *Gb = pass == 0 ? Gp : Gc;
+                SCF_PK(pk_fock_2e(ctx, Pb, Gb, PK_MEM_DEVICE));
+                pk_stats s2;
+                pk_ctx_stats(ctx, &s2);
+                for (int i = 0; i < nmol; ++i) res[i].fock_ms += s2.last_build_ms;
+                for (int i : gm)
+                    if ((rc = first_error(mols[i], Pb + lay_off[i], Gb + lay_off[i], pass == 0 ? eg[i] : ec[i])) !=
+                        PK_OK)
+                        goto done;
+            }
+            for (int i : gm) {
+                res[i].guess_diis_error = eg[i];
+                res[i].core_diis_error = ec[i];
+                if (ec[i] < eg[i]) {
+                    SCF_HIP(hipMemcpyAsync(Pp + lay_off[i], Pc + lay_off[i], mols[i].n2 * 8, hipMemcpyDeviceToDevice, st));
+                    SCF_HIP(hipMemcpyAsync(Gp + lay_off[i], Gc + lay_off[i], mols[i].n2 * 8, hipMemcpyDeviceToDevice, st));
+                    res[i].guess_used = 0;
+                }
+            }
+            have_G = true;
+        }
+    }
     // Iterations.  Every step of every unconverged molecule runs as a few
     // launches over molecule tables (pointer arrays and sizes staged in one
     // host buffer and copied in one transfer): F = H + G with the energy
@@ -961,13 +1022,14 @@ int pk_scf_rhf_batch(const pk_basis *const *bases, int nmol, const pk_scf_option
         };
         const dim3 blk(256);
         for (int it = 1; it <= max_cycles; ++it) {
-            SCF_PK(pk_fock_2e(ctx, Pp, Gp, PK_MEM_DEVICE));
             double build_ms = 0;
-            {
+            if (!have_G) {
+                SCF_PK(pk_fock_2e(ctx, Pp, Gp, PK_MEM_DEVICE));
                 pk_stats s2;
                 pk_ctx_stats(ctx, &s2);
                 build_ms = s2.last_build_ms;
             }
+            have_G = false;
             std::vector<int> act;
             for (int i = 0; i < nmol; ++i)
                 if (!mols[i].done) act.push_back(i);

@@ -188,13 +188,24 @@ class LearnedGuess:
         from .mlp import Net
         self.intra = Net(N_INTRA_IN, N_INTRA_OUT, hidden, max_batch, seed)
         self.inter = Net(N_INTER_IN, N_INTER_OUT, hidden, max_batch, seed + 1)
+        # the networks run on torch's current stream, so the batches that torch
+        # gathers (Xs[idx]) are complete before a kernel reads them
+        self.bind_stream()
         self.stats = None
+
+    def bind_stream(self, stream=None):
+        """Run both networks on `stream` (default: torch's current stream)."""
+        import torch
+        s = stream if stream is not None else torch.cuda.current_stream()
+        self.intra.set_stream(s.cuda_stream)
+        self.inter.set_stream(s.cuda_stream)
 
     def fit(self, samples, epochs=200, batch=512, lr=2e-3, log=None, seed=0):
         """samples: list of (X Angstrom, converged P).  Standardises inputs and
         outputs, then Adam on mini-batches, all on the device."""
         import torch
         dev = torch.device("cuda:0")
+        self.bind_stream()
         XI, YI, XE, YE = [], [], [], []
         for X, P in samples:
             fi, pairs, fe, O, R = features(X)
@@ -225,6 +236,7 @@ class LearnedGuess:
     def predict_blocks(self, X):
         import torch
         dev = torch.device("cuda:0")
+        self.bind_stream()
         fi, pairs, fe, O, R = features(X)
         out = []
         for net, F, kx, ky in ((self.intra, fi, "xi", "yi"), (self.inter, fe, "xe", "ye")):

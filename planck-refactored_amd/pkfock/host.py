@@ -118,9 +118,11 @@ class Molecule:
 
     @staticmethod
     def rhf_batch(mols, engine="os", tol_eri=1e-10, tol_energy=1e-10, tol_density=1e-10, diis_dim=8, max_cycles=0,
-                  guesses=None, fp32_cutoff=None):
+                  guesses=None, fp32_cutoff=None, guess_fallback=False):
         """Batched RHF of independent molecules (pk_scf_rhf_batch): one batched
-        Fock build per iteration for all of them.  Returns a list of result dicts
+        Fock build per iteration for all of them.  guess_fallback: each
+        molecule given a guess starts from the core guess instead when that
+        has the smaller first DIIS error.  Returns a list of result dicts
         (with "P")."""
         lib = load()
         o = ScfOptions()
@@ -131,6 +133,7 @@ class Molecule:
         o.charge = 0
         if fp32_cutoff is not None:
             o.precision, o.fp32_cutoff = _eng.PREC_MIXED, float(fp32_cutoff)
+        o.guess_fallback = int(bool(guess_fallback))
         nm = len(mols)
         res = (ScfResult * nm)()
         Ps = [np.zeros((m.nbasis, m.nbasis)) for m in mols]

@@ -86,12 +86,33 @@ class Net:
         _check(load().pk_mlp_get_grads(self.h, out.ctypes.data), "pk_mlp_get_grads")
         return out
 
+    def _device_rows(self, t, ncol, name, rows=None):
+        """The kernels take raw device pointers: refuse anything but a
+        contiguous float32 HIP tensor [rows][ncol] (a host pointer handed to
+        a kernel would be dereferenced on the device)."""
+        if not getattr(t, "is_cuda", False):
+            raise PkMlpError("%s must be a device (HIP) tensor" % name)
+        if str(t.dtype) != "torch.float32":
+            raise PkMlpError("%s must be float32, got %s" % (name, t.dtype))
+        if t.dim() != 2 or t.shape[1] != ncol:
+            raise PkMlpError("%s must have shape [batch][%d], got %s" % (name, ncol, tuple(t.shape)))
+        if not t.is_contiguous():
+            raise PkMlpError("%s must be contiguous" % name)
+        if rows is not None and t.shape[0] != rows:
+            raise PkMlpError("%s has %d rows, expected %d" % (name, t.shape[0], rows))
+        if t.shape[0] > self.max_batch:
+            raise PkMlpError("%s has %d rows, above max_batch %d" % (name, t.shape[0], self.max_batch))
+
     def forward(self, X, Y):
         """X, Y: contiguous float32 cuda tensors [batch][n_in], [batch][n_out]."""
+        self._device_rows(X, self.n_in, "X")
+        self._device_rows(Y, self.n_out, "Y", X.shape[0])
         _check(load().pk_mlp_forward(self.h, ctypes.c_void_p(X.data_ptr()), X.shape[0],
                                      ctypes.c_void_p(Y.data_ptr())), "pk_mlp_forward")
 
     def loss_grad(self, X, T):
+        self._device_rows(X, self.n_in, "X")
+        self._device_rows(T, self.n_out, "T", X.shape[0])
         loss = ctypes.c_float()
         _check(load().pk_mlp_loss_grad(self.h, ctypes.c_void_p(X.data_ptr()), ctypes.c_void_p(T.data_ptr()),
                                        X.shape[0], ctypes.byref(loss)), "pk_mlp_loss_grad")

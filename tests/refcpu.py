@@ -41,10 +41,12 @@ def lib():
         L.pkref_eri_full.argtypes = [vp, vp]
         L.pkref_fock_jk.argtypes = [vp, vp, vp, vp, ip]
         L.pkref_fock_jk_sample.argtypes = [vp, vp, vp, vp, ip, ctypes.c_long, ctypes.c_long, vp]
+        L.pkref_jk_elements.argtypes = [vp, vp, ip, vp, vp, vp, vp, ip]
         L.pkref_rhf.argtypes = [vp, ip, d, d, ip, ip, vp, vp, vp, vp, ip]
         L.pkref_boys.restype = d
         L.pkref_boys.argtypes = [ip, d]
         L.pkref_set_boys_exact.argtypes = [ip]
+        L.pkref_rys_rule.restype = ip
         L.pkref_rys_rule.argtypes = [ip, d, vp, vp]
         L.pkref_auto_prefers_rys.argtypes = [vp, vp, vp, vp]
         L.pkref_syev.argtypes = [ip, vp, vp, vp]
@@ -136,6 +138,17 @@ class Ref:
         lib().pkref_fock_jk_sample(self.ctx, _p(P), _p(J), _p(K), nthreads, stride, offset, ctypes.byref(nq))
         return J, K, nq.value
 
+    def jk_elements(self, P, rows, cols, nthreads=0):
+        """J[rows[e], cols[e]] and K[rows[e], cols[e]] of density P."""
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        r = np.ascontiguousarray(rows, dtype=np.int32)
+        c = np.ascontiguousarray(cols, dtype=np.int32)
+        J = np.zeros(len(r))
+        K = np.zeros(len(r))
+        if lib().pkref_jk_elements(self.ctx, _p(P), len(r), _p(r), _p(c), _p(J), _p(K), nthreads) != 0:
+            raise ValueError(lib().pkref_last_error().decode())
+        return J, K
+
     def rhf(self, charge=0, tol_energy=1e-10, tol_density=1e-10, diis_dim=8, max_cycles=0):
         E, it = ctypes.c_double(), ctypes.c_int()
         n = self.nbasis
@@ -165,7 +178,8 @@ def boys(n, x):
 
 
 def rys_rule(n, T):
-    r = np.zeros(n)
-    w = np.zeros(n)
-    lib().pkref_rys_rule(n, T, _p(r), _p(w))
+    r = np.zeros(max(n, 1))
+    w = np.zeros(max(n, 1))
+    if lib().pkref_rys_rule(n, T, _p(r), _p(w)) != 0:
+        raise ValueError(lib().pkref_last_error().decode())
     return r, w

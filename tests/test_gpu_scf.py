@@ -136,3 +136,26 @@ def test_batched_scf_matches_single_molecule_scf():
     # converged densities as guesses: every molecule stops after two iterations
     again = Molecule.rhf_batch(mols, guesses=[b["P"] for b in batch], **scf)
     assert all(a["iterations"] <= 3 and abs(a["energy"] - b["energy"]) < 1e-9 for a, b in zip(again, batch))
+
+
+def test_batched_scf_guess_fallback():
+    """Batched RHF with guess_fallback: a deliberately poor guess (a scaled
+    identity) for one molecule falls back to the core guess, a converged
+    density is kept for the others; every energy equals the plain batched
+    result within 1e-9 Eh."""
+    from pkfock.molecules import water_cluster
+    mols = [Molecule(*water_cluster(310 + s, n_min=2, n_max=4), "6-31g*") for s in range(4)]
+    scf = dict(tol_eri=1e-12, tol_energy=1e-9, tol_density=1e-7)
+    plain = Molecule.rhf_batch(mols, **scf)
+    guesses = [b["P"] for b in plain]
+    guesses[1] = 3.0 * np.eye(mols[1].nbasis)
+    guesses[2] = None
+    fb = Molecule.rhf_batch(mols, guesses=guesses, guess_fallback=True, **scf)
+    for i, (p, f) in enumerate(zip(plain, fb)):
+        print("molecule %d: guess_used %d  errors %.3e / %.3e  %d it  dE %.2e" % (
+            i, f["guess_used"], f["guess_diis_error"], f["core_diis_error"], f["iterations"],
+            f["energy"] - p["energy"]))
+        assert f["rc"] == 0 and abs(f["energy"] - p["energy"]) < 1e-9
+    assert fb[1]["guess_used"] == 0 and fb[1]["core_diis_error"] < fb[1]["guess_diis_error"]
+    assert fb[0]["guess_used"] == 1 and fb[3]["guess_used"] == 1 and fb[0]["iterations"] <= 3
+    assert fb[2]["guess_used"] == 0

@@ -101,3 +101,44 @@ def test_mlp_oracle_adam_first_step_is_signed_lr():
     g = np.array([0.3, -2.0, 1e-3])
     p, m1, m2 = mlp_ref.adam(np.zeros(3), g, np.zeros(3), np.zeros(3), 1, lr=0.01)
     assert np.abs(p + 0.01 * g / (np.abs(g) + 1e-8)).max() < 1e-12
+
+
+def test_mlp_binding_refuses_host_and_misshapen_tensors():
+    """The network kernels take raw device pointers: the binding must refuse a
+    host tensor, another dtype, a wrong width, a strided view and row-count
+    mismatches before any pointer reaches the library."""
+    import types
+
+    import torch
+    from pkfock.mlp import Net, PkMlpError
+
+    fake = types.SimpleNamespace(max_batch=8)
+    check = Net._device_rows
+    with pytest.raises(PkMlpError, match="device"):
+        check(fake, torch.zeros(4, 3, dtype=torch.float32), 3, "X")
+
+    class FakeDev:  # stands in for a device tensor's metadata on a CPU-only host
+        def __init__(self, t):
+            self.t = t
+            self.is_cuda = True
+            self.dtype, self.shape = t.dtype, t.shape
+
+        def dim(self):
+            return self.t.dim()
+
+        def is_contiguous(self):
+            return self.t.is_contiguous()
+
+    ok = FakeDev(torch.zeros(4, 3, dtype=torch.float32))
+    check(fake, ok, 3, "X")
+    check(fake, ok, 3, "X", rows=4)
+    with pytest.raises(PkMlpError, match="float32"):
+        check(fake, FakeDev(torch.zeros(4, 3, dtype=torch.float64)), 3, "X")
+    with pytest.raises(PkMlpError, match="shape"):
+        check(fake, ok, 5, "X")
+    with pytest.raises(PkMlpError, match="contiguous"):
+        check(fake, FakeDev(torch.zeros(3, 4, dtype=torch.float32).t()), 3, "X")
+    with pytest.raises(PkMlpError, match="rows"):
+        check(fake, ok, 3, "T", rows=5)
+    with pytest.raises(PkMlpError, match="max_batch"):
+        check(fake, FakeDev(torch.zeros(9, 3, dtype=torch.float32)), 3, "X")

@@ -209,6 +209,25 @@ def test_rys_rule_moments():
                 assert abs(float(np.sum(w * r ** k)) - ex) <= 1e-11 * ex, (n, T, k)
 
 
+def test_rys_rule_refused_beyond_its_pins():
+    """The reference's moment method loses accuracy from n = 6 roots: the
+    oracle refuses it there and its Rys engine falls back to OS, so a (dd|dd)
+    ERI (n = 5) stays Rys and an (fd|dd) component (n = 6) equals OS."""
+    with pytest.raises(ValueError):
+        refcpu.rys_rule(6, 1.0)
+    with pytest.raises(ValueError):
+        refcpu.rys_rule(0, 1.0)
+    Z, X = [6, 8], np.array([[0.0, 0.0, 0.0], [0.0, 0.0, 2.1316]])
+    ros = refcpu.Ref("cc-pvtz", Z, X, engine="os")
+    rrys = refcpu.Ref("cc-pvtz", Z, X, engine="rys")
+    n = ros.nbasis
+    # the last functions of each atom's list are its f shells (file order)
+    f_c = [i for i in range(n) if i < 35][-10:]
+    for (i, j, k, l) in [(f_c[0], f_c[3], f_c[5], f_c[9]), (f_c[1], 0, f_c[2], 1)]:
+        a, b = ros.eri(i, j, k, l), rrys.eri(i, j, k, l)
+        assert abs(a - b) <= 1e-13 * max(1.0, abs(a)), (i, j, k, l, a, b)
+
+
 def test_auto_routing_fractions():
     """Per-class share of Cartesian components the reference's cost model sends
     to Rys (orientation la >= lb, lc >= ld)."""
