@@ -138,6 +138,21 @@ int pk_schwarz(pk_ctx *ctx, double *Q, int memkind);
  * when nb^4 * 8 bytes exceeds 2 GiB. */
 int pk_eri_tensor(pk_ctx *ctx, double *out);
 
+/* Two-electron part of the nuclear gradient: the contraction of the ERI first
+ * derivatives (ObaraSaika::_compute_eri_deriv_elem, src/integrals/os.cpp:1296-1420)
+ * with the two-particle density in the ERI loop of the reference's gradients
+ * (src/gradient/gradient.cpp:209-232 closed shell, :436-463 UHF):
+ *   grad[3 s + q] = dE2 / d(centre of shell s)_q,  E2 = 1/4 sum_ijkl Gamma_ijkl (ij|kl),
+ *   Gamma_ijkl = 2 Pj_ij Pj_kl - kx (Pk1_ik Pk1_jl + Pk2_ik Pk2_jl).
+ * RHF: Pj = Pk1 = P (total density), Pk2 = NULL, kx = 1; RKS: kx = the exact-
+ * exchange fraction (gradient.cpp:307-310); UHF: Pj = Pa + Pb, Pk1 = Pa,
+ * Pk2 = Pb, kx = 2 (gradient.cpp:428-434).  grad has 3 * nshells entries in the
+ * caller's shell order; the caller adds the shells of each atom (the
+ * reference's build_shell_atom_map, gradient.cpp:20-46).  FP64, the context's
+ * tol_eri screening; one molecule; host or device pointers. */
+int pk_eri_gradient(pk_ctx *ctx, const double *Pj, const double *Pk1, const double *Pk2, double kx, double *grad,
+                    int memkind);
+
 /* Per-class device timing (HIP events around each class's kernels on the
  * context stream).  Disabled by default; enabling resets the counters. */
 typedef struct pk_class_stats {

@@ -1363,6 +1363,203 @@ int pkref_jk_elements(pkref_ctx *c, const double *P, int nel, const int *ei, con
     return 0;
 }
 
+// ERI first derivatives of one contracted Cartesian quartet (ij|kl), restating
+// ObaraSaika::_compute_eri_deriv_elem (src/integrals/os.cpp:1296-1420): for
+// centre X in (A, B, C, D) of the functions (i, j, k, l) and axis q,
+//   d(ij|kl)/dX_q = sum_prims 2 x_exp c_ij c_kl [.. x+1_q ..] - x_q (.. x-1_q ..)
+// where x_exp is the primitive exponent on that centre (alpha, beta of the bra
+// pair, alpha, beta of the ket pair), [..] the primitive OS ERI with the
+// centre's Cartesian exponent raised along q and (..) the contracted ERI with
+// it lowered (absent when x_q = 0).  out[cen * 3 + q], cen 0..3 = A, B, C, D.
+static void eri_deriv_elem(const pkref_ctx &c, const Pair &ab, const Pair &cd, double out[12]) {
+    const Func &A = c.funcs[ab.i], &B = c.funcs[ab.j], &C = c.funcs[cd.i], &D = c.funcs[cd.j];
+    for (int x = 0; x < 12; ++x) out[x] = 0.0;
+    for (int cen = 0; cen < 4; ++cen)
+        for (int q = 0; q < 3; ++q) {
+            int l[4][3];
+            for (int t = 0; t < 3; ++t) {
+                l[0][t] = A.l[t];
+                l[1][t] = B.l[t];
+                l[2][t] = C.l[t];
+                l[3][t] = D.l[t];
+            }
+            const int lq = l[cen][q];
+            l[cen][q] = lq + 1;
+            double up = 0.0;
+            for (const PrimPair &p : ab.pp)
+                for (const PrimPair &r : cd.pp) {
+                    const double w = cen == 0 ? p.alpha : cen == 1 ? p.beta : cen == 2 ? r.alpha : r.beta;
+                    up += 2.0 * w * p.coeff * r.coeff * os_eri_primitive(p, r, l[0], l[1], l[2], l[3], ab.R, cd.R);
+                }
+            double down = 0.0;
+            if (lq > 0) {
+                l[cen][q] = lq - 1;
+                for (const PrimPair &p : ab.pp)
+                    for (const PrimPair &r : cd.pp)
+                        down += p.coeff * r.coeff * os_eri_primitive(p, r, l[0], l[1], l[2], l[3], ab.R, cd.R);
+            }
+            out[cen * 3 + q] = up - (double)lq * down;
+        }
+}
+
+// Two-electron part of the nuclear gradient, restating the ERI loop of
+// compute_closed_shell_gradient_from_density / compute_uhf_gradient
+// (src/gradient/gradient.cpp:209-232, 436-463) with
+// accumulate_eri_gradient_permutations (gradient.cpp:70-118): every ordered
+// pair of AO pairs (i <= j, k <= l) passing Q_ij Q_kl >= tol contributes
+// 0.25 Gamma d(ij|kl)/dX to the atom of each centre, for the index
+// permutations (ij|kl), (ij|lk), (ji|kl), (ji|lk) that are distinct, each
+// skipped when |Gamma| < 1e-14 as in the reference, with
+//   Gamma_ijkl = 2 Pj_ij Pj_kl - kx (Pk1_ik Pk1_jl + Pk2_ik Pk2_jl)
+// (RHF: Pj = Pk1 = P, no Pk2, kx = 1; RKS: kx = the exact-exchange fraction;
+// UHF: Pj = Pa + Pb, Pk1 = Pa, Pk2 = Pb, kx = 2).  grad is natoms x 3
+// (row-major), overwritten.  Deterministic: fixed slots summed in order.
+static int eri_gradient(pkref_ctx *c, const double *Pj, const double *Pk1, const double *Pk2, double kx,
+                        double *grad, int nthreads) {
+    ensure_schwarz(*c);
+    const int nb = c->nb, nat = (int)c->Z.size();
+    const long np = (long)c->pairs.size();
+    const int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+    auto gamma = [&](size_t i, size_t j, size_t k, size_t l) {
+        double g = 2.0 * Pj[i * nb + j] * Pj[k * nb + l] - kx * Pk1[i * nb + k] * Pk1[j * nb + l];
+        if (Pk2) g -= kx * Pk2[i * nb + k] * Pk2[j * nb + l];
+        return g;
+    };
+    const int nslot = (int)std::min<long>(64, std::max<long>(1, np));
+    std::vector<std::vector<double>> acc(nslot, std::vector<double>((size_t)nat * 3, 0.0));
+#pragma omp parallel for num_threads(nt) schedule(dynamic)
+    for (int sl = 0; sl < nslot; ++sl) {
+        std::vector<double> &g = acc[sl];
+        for (long p = sl; p < np; p += nslot) {
+            const Pair &ab = c->pairs[p];
+            const size_t i = ab.i, j = ab.j;
+            const int atA = c->shells[c->funcs[i].shell].atom, atB = c->shells[c->funcs[j].shell].atom;
+            for (long q = 0; q < np; ++q) {
+                const Pair &cd = c->pairs[q];
+                const size_t k = cd.i, l = cd.j;
+                if (c->Q[i * nb + j] * c->Q[k * nb + l] < c->tol) continue;
+                const int atC = c->shells[c->funcs[k].shell].atom, atD = c->shells[c->funcs[l].shell].atom;
+                double dI[12];
+                eri_deriv_elem(*c, ab, cd, dI);
+                auto perm = [&](double gm, bool sab, bool scd) {
+                    if (std::fabs(gm) < 1e-14) return;
+                    const int da = sab ? 1 : 0, db = sab ? 0 : 1, dc = scd ? 3 : 2, dd = scd ? 2 : 3;
+                    const int xa = sab ? atB : atA, xb = sab ? atA : atB, xc = scd ? atD : atC, xd = scd ? atC : atD;
+                    const double f = 0.25 * gm;
+                    for (int t = 0; t < 3; ++t) {
+                        g[xa * 3 + t] += f * dI[da * 3 + t];
+                        g[xb * 3 + t] += f * dI[db * 3 + t];
+                        g[xc * 3 + t] += f * dI[dc * 3 + t];
+                        g[xd * 3 + t] += f * dI[dd * 3 + t];
+                    }
+                };
+                perm(gamma(i, j, k, l), false, false);
+                if (k != l) perm(gamma(i, j, l, k), false, true);
+                if (i != j) perm(gamma(j, i, k, l), true, false);
+                if (i != j && k != l) perm(gamma(j, i, l, k), true, true);
+            }
+        }
+    }
+    for (int x = 0; x < nat * 3; ++x) {
+        double s = 0.0;
+        for (int t = 0; t < nslot; ++t) s += acc[t][x];
+        grad[x] = s;
+    }
+    return 0;
+}
+
+// Primitive-pair screening of the engine (not reference behaviour: the
+// reference keeps every primitive pair), restated so that its effect can be
+// measured against the unscreened oracle.  A primitive pair is a primitive of
+// one shell times a primitive of another (or the same) shell; it is dropped
+// from every function pair of those two shells when
+//   rule 0: |kc_p| * max_q |kc_q| / sqrt(zeta_q) < cut,
+//           kc = (pi/zeta)^1.5 exp(-alpha beta R^2/zeta) c_a N_a c_b N_b sqrt(zeta) (4/pi)^(1/4)
+//           (a prefactor bound that leaves out the angular factors);
+//   rule 1: Q_p * max_q Q_q < cut, Q_p = max over the components a, b of the
+//           two shells of sqrt|[ab|ab]| of the single primitive pair, with
+//           c_a N_a c_b N_b and no component norms (the primitive Schwarz
+//           bound: no primitive quartet moves a contracted integral by more
+//           than Q_p Q_q).
+// The Schwarz table is recomputed from the remaining primitives.  Returns
+// the number of primitive pairs (of shell pairs) dropped.
+int pkref_screen_primitives(pkref_ctx *c, double cut, int rule) {
+    struct Key {
+        int s0, s1, p0, p1;
+        bool operator<(const Key &o) const {
+            return s0 != o.s0 ? s0 < o.s0 : s1 != o.s1 ? s1 < o.s1 : p0 != o.p0 ? p0 < o.p0 : p1 < o.p1;
+        }
+    };
+    auto key = [&](const Pair &pr, size_t k) {
+        const Func &fa = c->funcs[pr.i], &fb = c->funcs[pr.j];
+        const int na = (int)c->shells[fa.shell].e.size(), nbp = (int)c->shells[fb.shell].e.size();
+        (void)na;
+        int ia = (int)(k / nbp), ib = (int)(k % nbp), sa = fa.shell, sb = fb.shell;
+        if (sa > sb || (sa == sb && ia > ib)) {
+            std::swap(sa, sb);
+            std::swap(ia, ib);
+        }
+        return Key{sa, sb, ia, ib};
+    };
+    std::map<Key, double> bound;  // per primitive pair of shells
+    for (const Pair &pr : c->pairs) {
+        const Func &fa = c->funcs[pr.i], &fb = c->funcs[pr.j];
+        for (size_t k = 0; k < pr.pp.size(); ++k) {
+            const PrimPair &pp = pr.pp[k];
+            const double raw = pp.coeff / (fa.cnorm * fb.cnorm);
+            double v;
+            if (rule == 0) {
+                v = std::fabs(pp.prefactor * raw) * std::sqrt(pp.zeta) * 1.0622519320271969145;
+            } else {
+                PrimPair q = pp;
+                v = std::sqrt(std::fabs(raw * raw * os_eri_primitive(q, q, fa.l, fb.l, fa.l, fb.l, pr.R, pr.R)));
+            }
+            double &b = bound[key(pr, k)];
+            if (rule == 0) {
+                b = v;  // the same for every component pair
+            } else {
+                b = std::max(b, v);
+            }
+        }
+    }
+    double partner = 0.0;  // max over primitive pairs of the partner factor
+    for (const Pair &pr : c->pairs)
+        for (size_t k = 0; k < pr.pp.size(); ++k) {
+            const double b = bound[key(pr, k)];
+            partner = std::max(partner, rule == 0 ? b / std::sqrt(pr.pp[k].zeta) : b);
+        }
+    int dropped = 0;
+    for (const auto &kv : bound)
+        if (kv.second * partner < cut) ++dropped;
+    for (Pair &pr : c->pairs) {
+        std::vector<PrimPair> keep;
+        for (size_t k = 0; k < pr.pp.size(); ++k)
+            if (bound[key(pr, k)] * partner >= cut) keep.push_back(pr.pp[k]);
+        pr.pp.swap(keep);
+    }
+    c->have_q = false;
+    return dropped;
+}
+
+int pkref_eri_deriv(pkref_ctx *c, int i, int j, int k, int l, double *out12) {
+    const int nb = c->nb;
+    if (i < 0 || j < 0 || k < 0 || l < 0 || i >= nb || j >= nb || k >= nb || l >= nb || i > j || k > l) {
+        g_err = "pkref_eri_deriv: indices must satisfy 0 <= i <= j < nb, 0 <= k <= l < nb";
+        return -1;
+    }
+    eri_deriv_elem(*c, c->pairs[pair_index(nb, i, j)], c->pairs[pair_index(nb, k, l)], out12);
+    return 0;
+}
+
+int pkref_eri_gradient(pkref_ctx *c, const double *Pj, const double *Pk1, const double *Pk2, double kx,
+                       double *grad, int nthreads) {
+    if (!Pj || !Pk1 || !grad) {
+        g_err = "pkref_eri_gradient: null density or output";
+        return -1;
+    }
+    return eri_gradient(c, Pj, Pk1, Pk2, kx, grad, nthreads);
+}
+
 double pkref_boys(int n, double x) {
     init_boys_table();
     return boys(n, x);

@@ -56,6 +56,21 @@ int pkref_fock_jk_sample(pkref_ctx *ctx, const double *P, double *J, double *K,
 int pkref_jk_elements(pkref_ctx *ctx, const double *P, int nel, const int *ei, const int *ej, double *J,
                       double *K, int nthreads);
 
+/* 12 first derivatives d(ij|kl)/dX_q of one contracted ERI, i <= j, k <= l,
+ * out[cen*3 + q] with cen 0..3 = centres of i, j, k, l (os.cpp:1296-1420). */
+int pkref_eri_deriv(pkref_ctx *ctx, int i, int j, int k, int l, double *out12);
+/* Two-electron gradient, natoms x 3 row-major (gradient.cpp:209-232, 70-118):
+ * 1/4 sum Gamma_ijkl d(ij|kl)/dR with
+ * Gamma = 2 Pj_ij Pj_kl - kx (Pk1_ik Pk1_jl + Pk2_ik Pk2_jl); Pk2 may be NULL. */
+int pkref_eri_gradient(pkref_ctx *ctx, const double *Pj, const double *Pk1, const double *Pk2, double kx,
+                       double *grad, int nthreads);
+
+/* Primitive-pair screening of the engine (not the reference's behaviour),
+ * for measuring its effect: rule 0 the prefactor bound, rule 1 the primitive
+ * Schwarz bound (pk_ref.cpp).  Returns the number of shell primitive pairs
+ * dropped; the Schwarz table is recomputed afterwards. */
+int pkref_screen_primitives(pkref_ctx *ctx, double cut, int rule);
+
 /* RHF SCF exactly as run_rhf (core guess).  hist gets E_total per iteration. */
 int pkref_rhf(pkref_ctx *ctx, int charge, double tol_energy, double tol_density,
               int diis_dim, int max_cycles, double *energy, int *iterations,

@@ -25,6 +25,7 @@
 
 #include "eri_classes.h"
 #include "eri_generic.h"
+#include "eri_grad.h"
 
 // per-class launchers, defined in pk_kernels.hip
 #define PK_DECL(a, b, c, d, shard)                                                             \
@@ -220,12 +221,16 @@ struct pk_ctx {
     std::vector<int> pair_cls;
     std::vector<double> pair_q;
     std::vector<pk::PrimDev> prims;
+    std::vector<double> palpha;  // exponent on the pair's first shell, per primitive pair
+    std::vector<int> pair_sh;    // first and second shell of each pair
     std::vector<double> Q;  // packed nb x nb blocks, normalised functions, internal order
     pk::PairDev *d_pairs = nullptr;
     pk::PrimDev *d_prims = nullptr;
     double *d_Q = nullptr, *d_fnorm = nullptr, *d_P = nullptr, *d_Pn = nullptr, *d_A = nullptr, *d_B = nullptr;
     double *d_J = nullptr, *d_K = nullptr;
     double *d_J2 = nullptr;  // UHF: J of the second density
+    double *d_palpha = nullptr, *d_Pk1n = nullptr, *d_Pk2n = nullptr, *d_gsh = nullptr;  // gradient
+    int *d_pair_sh = nullptr;
     std::vector<ClassWork> work;
     bool work_ok = false;
     int engine = PK_ENGINE_OS;
@@ -524,22 +529,31 @@ int ensure_buffers(pk_ctx *c) {
 // Enqueue the kernels of one build on the context stream: P (device) -> A, B
 // accumulators -> J, K, G (any may be null).  Class events only when profiling
 // (never while the stream is being captured).
-int launch_build(pk_ctx *c, const double *dP, double *dJ, double *dK, double *dG) {
+// Per-build scratch: the scaled density and the J, K accumulators (packed,
+// c->n2 doubles each).
+struct BuildBufs {
+    double *Pn, *A, *B;
+};
+
+BuildBufs ctx_bufs(pk_ctx *c) { return BuildBufs{c->d_Pn, c->d_A, c->d_B}; }
+
+int launch_build(pk_ctx *c, const double *dP, double *dJ, double *dK, double *dG, const BuildBufs &sb,
+                 bool events) {
     int rc;
     hipStream_t s = c->stream;
     const MolTab t{c->d_mol_off, c->d_mol_nb, c->d_mol_foff, c->d_fpos};
     const dim3 g2((unsigned)(((size_t)c->nbmax * c->nbmax + 255) / 256), (unsigned)c->nmol);
     const int gf = (int)((c->n2 + 255) / 256);
-    hipLaunchKernelGGL(k_scale_density, g2, dim3(256), 0, s, dP, c->d_fnorm, c->d_Pn, t);
-    hipLaunchKernelGGL(k_fill, dim3(gf), dim3(256), 0, s, c->d_A, c->n2, 0.0);
-    hipLaunchKernelGGL(k_fill, dim3(gf), dim3(256), 0, s, c->d_B, c->n2, 0.0);
+    hipLaunchKernelGGL(k_scale_density, g2, dim3(256), 0, s, dP, c->d_fnorm, sb.Pn, t);
+    hipLaunchKernelGGL(k_fill, dim3(gf), dim3(256), 0, s, sb.A, c->n2, 0.0);
+    hipLaunchKernelGGL(k_fill, dim3(gf), dim3(256), 0, s, sb.B, c->n2, 0.0);
     int nl = 3;
     for (ClassWork &w : c->work) {
         const int *wv = w.d_wave;
         pk::JKArgs a{wv, wv + w.nwaves, wv + 2 * w.nwaves, wv + 3 * w.nwaves, w.d_kets, c->d_pairs,
-                     c->d_prims, c->d_mol_off, c->d_mol_nb, c->d_Pn, c->d_Q, c->d_A, c->d_B,
+                     c->d_prims, c->d_mol_off, c->d_mol_nb, sb.Pn, c->d_Q, sb.A, sb.B,
                      c->tol, w.nwaves, w.remap};
-        if (c->profile) {
+        if (events) {
             if (!w.e0) PK_HIP(hipEventCreate(&w.e0));
             if (!w.e1) PK_HIP(hipEventCreate(&w.e1));
             PK_HIP(hipEventRecord(w.e0, s));
@@ -579,9 +593,9 @@ int launch_build(pk_ctx *c, const double *dP, double *dJ, double *dK, double *dG
             ++nl;
         }
         w.kernels = nl - before;
-        if (c->profile) PK_HIP(hipEventRecord(w.e1, s));
+        if (events) PK_HIP(hipEventRecord(w.e1, s));
     }
-    hipLaunchKernelGGL(k_finalize, g2, dim3(256), 0, s, c->d_A, c->d_B, c->d_fnorm, dJ, dK, dG, t);
+    hipLaunchKernelGGL(k_finalize, g2, dim3(256), 0, s, sb.A, sb.B, c->d_fnorm, dJ, dK, dG, t);
     ++nl;
     PK_HIP(hipGetLastError());
     c->st.kernel_launches = nl;
@@ -600,14 +614,14 @@ int build_device(pk_ctx *c, const double *dP, double *dJ, double *dK, double *dG
     const size_t bytes = c->n2 * sizeof(double);
     if (!c->graph_on || c->profile) {
         PK_HIP(hipEventRecord(c->ev0, s));
-        if ((rc = launch_build(c, dP, dJ, dK, dG)) != PK_OK) return rc;
+        if ((rc = launch_build(c, dP, dJ, dK, dG, ctx_bufs(c), c->profile)) != PK_OK) return rc;
         PK_HIP(hipEventRecord(c->ev1, s));
         return PK_OK;
     }
     if (!c->graph_ok) {
         drop_graph(c);
         PK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        const int rc_cap = launch_build(c, c->d_P, c->d_J, c->d_K, c->d_G);
+        const int rc_cap = launch_build(c, c->d_P, c->d_J, c->d_K, c->d_G, ctx_bufs(c), false);
         hipGraph_t g = nullptr;
         const hipError_t e = hipStreamEndCapture(s, &g);
         if (rc_cap != PK_OK) {
@@ -752,6 +766,7 @@ int add_molecule(pk_ctx *c, const pk_basis_desc *d, int m) {
     // shell pairs a >= b, first shell carries the larger l (types.h:606-675 pair data)
     const size_t pair0 = c->pairs.size();
     std::vector<std::vector<pk::PrimDev>> pp_all;
+    std::vector<std::vector<double>> pa_all;
     double kmax = 0.0;  // max over primitive pairs of |kc| / sqrt(zeta)
     for (int A = s0; A < c->ns; ++A)
         for (int Bs = s0; Bs <= A; ++Bs) {
@@ -764,6 +779,7 @@ int add_molecule(pk_ctx *c, const pk_basis_desc *d, int m) {
             for (int q = 0; q < 3; ++q) p.ab[q] = Ac[q] - Bc[q];
             const double R2 = p.ab[0] * p.ab[0] + p.ab[1] * p.ab[1] + p.ab[2] * p.ab[2];
             std::vector<pk::PrimDev> pps;
+            std::vector<double> pal;
             for (int i = 0; i < c->sh_np[sa]; ++i)
                 for (int j = 0; j < c->sh_np[sb]; ++j) {
                     const double al = c->ex[c->sh_poff[sa] + i], be = c->ex[c->sh_poff[sb] + j];
@@ -782,8 +798,12 @@ int add_molecule(pk_ctx *c, const pk_basis_desc *d, int m) {
                     }
                     kmax = std::max(kmax, std::fabs(pp.kc) / std::sqrt(pp.zeta));
                     pps.push_back(pp);
+                    pal.push_back(al);
                 }
             pp_all.push_back(std::move(pps));
+            pa_all.push_back(std::move(pal));
+            c->pair_sh.push_back(sa);
+            c->pair_sh.push_back(sb);
             c->pairs.push_back(p);
             c->pair_cls.push_back(pk::pair_class(c->sh_l[sa], c->sh_l[sb]));
             c->pair_q.push_back(0.0);
@@ -796,8 +816,11 @@ int add_molecule(pk_ctx *c, const pk_basis_desc *d, int m) {
     for (size_t k = 0; k < pp_all.size(); ++k) {
         pk::PairDev &p = c->pairs[pair0 + k];
         p.prim_off = (int)c->prims.size();
-        for (const pk::PrimDev &pp : pp_all[k])
-            if (std::fabs(pp.kc) * kmax >= kPrimCut) c->prims.push_back(pp);
+        for (size_t i = 0; i < pp_all[k].size(); ++i)
+            if (std::fabs(pp_all[k][i].kc) * kmax >= kPrimCut) {
+                c->prims.push_back(pp_all[k][i]);
+                c->palpha.push_back(pa_all[k][i]);
+            }
         p.nprim = (int)c->prims.size() - p.prim_off;
         p.grp = prim_group(p.nprim);
     }
@@ -838,6 +861,8 @@ int ctx_build(const pk_basis_desc *descs, int nmol, pk_ctx **out) {
         (rc = dev_upload(&c->d_mol_foff, c->mol_foff, c->stream)) != PK_OK ||
         (rc = dev_upload(&c->d_pairs, c->pairs, c->stream)) != PK_OK ||
         (rc = dev_upload(&c->d_prims, c->prims, c->stream)) != PK_OK ||
+        (rc = dev_upload(&c->d_palpha, c->palpha, c->stream)) != PK_OK ||
+        (rc = dev_upload(&c->d_pair_sh, c->pair_sh, c->stream)) != PK_OK ||
         (rc = dev_upload(&c->d_fnorm, c->fnorm, c->stream)) != PK_OK ||
         (rc = dev_upload(&c->d_fpos, c->fpos, c->stream)) != PK_OK || (rc = build_schwarz(c)) != PK_OK ||
         (rc = dev_upload(&c->d_Q, c->Q, c->stream)) != PK_OK ||
@@ -902,7 +927,7 @@ int pk_ctx_autotune(pk_ctx *c, int reps) {
         std::vector<double> t(nw, 0.0);
         for (int r = 0; r < nrep; ++r) {
             int rc2;
-            if ((rc2 = launch_build(c, c->d_P, c->d_J, c->d_K, nullptr)) != PK_OK) return rc2;
+            if ((rc2 = launch_build(c, c->d_P, c->d_J, c->d_K, nullptr, ctx_bufs(c), true)) != PK_OK) return rc2;
             PK_HIP(hipStreamSynchronize(s));
             for (size_t i = 0; i < nw; ++i) {
                 float ms = 0.f;
@@ -948,6 +973,7 @@ int pk_ctx_destroy(pk_ctx *c) {
     free_work(c);
     for (void *p : {(void *)c->d_pairs, (void *)c->d_prims, (void *)c->d_Q, (void *)c->d_fnorm, (void *)c->d_P,
                     (void *)c->d_Pn, (void *)c->d_A, (void *)c->d_B, (void *)c->d_J, (void *)c->d_K, (void *)c->d_G, (void *)c->d_J2,
+                    (void *)c->d_palpha, (void *)c->d_Pk1n, (void *)c->d_Pk2n, (void *)c->d_gsh, (void *)c->d_pair_sh,
                     (void *)c->d_mol_off, (void *)c->d_mol_nb, (void *)c->d_mol_foff, (void *)c->d_fpos})
         if (p) (void)hipFree(p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -1098,6 +1124,76 @@ int pk_fock_jk_uhf(pk_ctx *c, const double *Pa, const double *Pb, double *Jt, do
     if (Kb) PK_HIP(hipMemcpyAsync(Kb, c->d_K, n2, hipMemcpyDeviceToHost, s));
     PK_HIP(hipStreamSynchronize(s));
     return finish_timing(c);
+}
+
+// Two-electron gradient: the densities are scaled into the internal order,
+// then every class list of the energy build (same screening) runs through the
+// derivative kernel; the per-shell sums leave through the context's buffer.
+int pk_eri_gradient(pk_ctx *c, const double *Pj, const double *Pk1, const double *Pk2, double kx, double *grad,
+                    int memkind) {
+    if (!c || !Pj || !Pk1 || !grad) return fail(PK_ERR_ARG, "pk_eri_gradient: null argument");
+    if (memkind != PK_MEM_HOST && memkind != PK_MEM_DEVICE) return fail(PK_ERR_ARG, "pk_eri_gradient: bad memkind");
+    if (c->nmol != 1) return fail(PK_ERR_ARG, "pk_eri_gradient: one molecule only");
+    int rc;
+    if (!c->work_ok && (rc = build_work(c)) != PK_OK) return rc;
+    if ((rc = ensure_buffers(c)) != PK_OK) return rc;
+    const size_t n2 = c->n2 * sizeof(double);
+    if (!c->d_Pk1n) PK_HIP(hipMalloc((void **)&c->d_Pk1n, n2));
+    if (!c->d_Pk2n) PK_HIP(hipMalloc((void **)&c->d_Pk2n, n2));
+    if (!c->d_gsh) PK_HIP(hipMalloc((void **)&c->d_gsh, 3 * (size_t)c->ns * sizeof(double)));
+    static const hipError_t attr = hipFuncSetAttribute(
+        (const void *)pk::k_grad, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pk::grad_lds_bytes(kMaxL, kMaxL, kMaxL, kMaxL));
+    PK_HIP(attr);
+    hipStream_t s = c->stream;
+    const double *src[3] = {Pj, Pk1, Pk2};
+    if (memkind == PK_MEM_HOST) {
+        // staging through the context's P, J and K buffers
+        double *stage[3] = {c->d_P, c->d_J, c->d_K};
+        for (int i = 0; i < 3; ++i)
+            if (src[i]) {
+                PK_HIP(hipMemcpyAsync(stage[i], src[i], n2, hipMemcpyHostToDevice, s));
+                src[i] = stage[i];
+            }
+    }
+    const MolTab t{c->d_mol_off, c->d_mol_nb, c->d_mol_foff, c->d_fpos};
+    const dim3 g2((unsigned)(((size_t)c->nbmax * c->nbmax + 255) / 256), 1);
+    double *dst[3] = {c->d_Pn, c->d_Pk1n, c->d_Pk2n};
+    for (int i = 0; i < 3; ++i)
+        if (src[i]) hipLaunchKernelGGL(k_scale_density, g2, dim3(256), 0, s, src[i], c->d_fnorm, dst[i], t);
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)((3 * c->ns + 255) / 256)), dim3(256), 0, s, c->d_gsh,
+                       (size_t)3 * c->ns, 0.0);
+    for (const ClassWork &w : c->work) {
+        const ClassFns &f = kClasses[w.cls];
+        const int *wv = w.d_wave;
+        pk::GradArgs a{};
+        a.wave_bra = wv;
+        a.wave_start = wv + w.nwaves;
+        a.wave_count = wv + 2 * w.nwaves;
+        a.kets = w.d_kets;
+        a.pairs = c->d_pairs;
+        a.prims = c->d_prims;
+        a.palpha = c->d_palpha;
+        a.pair_sh = c->d_pair_sh;
+        a.Pj = c->d_Pn;
+        a.Pk1 = c->d_Pk1n;
+        a.Pk2 = Pk2 ? c->d_Pk2n : nullptr;
+        a.Q = c->d_Q;
+        a.g = c->d_gsh;
+        a.tol = c->tol;
+        a.kx = kx;
+        a.nb = c->nb;
+        a.nitems = w.nwaves;
+        a.la = f.la;
+        a.lb = f.lb;
+        a.lc = f.lc;
+        a.ld = f.ld;
+        hipLaunchKernelGGL(pk::k_grad, dim3(w.nwaves), dim3(64), pk::grad_lds_bytes(f.la, f.lb, f.lc, f.ld), s, a);
+    }
+    PK_HIP(hipGetLastError());
+    const size_t gb = 3 * (size_t)c->ns * sizeof(double);
+    PK_HIP(hipMemcpyAsync(grad, c->d_gsh, gb, memkind == PK_MEM_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, s));
+    PK_HIP(hipStreamSynchronize(s));
+    return PK_OK;
 }
 
 int pk_schwarz(pk_ctx *c, double *Q, int memkind) {

@@ -76,6 +76,7 @@ def load():
     lib.pk_ctx_mol_layout.argtypes = [vp, vp, vp]
     lib.pk_ctx_set_graph.argtypes = [vp, ip]
     lib.pk_ctx_autotune.argtypes = [vp, ip]
+    lib.pk_eri_gradient.argtypes = [vp, vp, vp, vp, ctypes.c_double, vp, ip]
     lib.pk_last_error.restype = ctypes.c_char_p
     _lib = lib
     return lib
@@ -118,6 +119,21 @@ class Shells:
         d.coef = self.coef.ctypes.data_as(dbl)
         d.prim_norm = self.prim_norm.ctypes.data_as(dbl)
         return d
+
+
+def shells_to_atoms(gshell, centers, xyz_bohr):
+    """Sum per-shell rows onto atoms: shell s belongs to the atom whose
+    position equals its centre within 1e-5 Bohr (the reference's
+    build_shell_atom_map, src/gradient/gradient.cpp:20-46)."""
+    xyz = np.asarray(xyz_bohr, dtype=np.float64).reshape(-1, 3)
+    out = np.zeros((len(xyz), gshell.shape[1]))
+    for s, c in enumerate(np.asarray(centers).reshape(-1, 3)):
+        d2 = ((xyz - c) ** 2).sum(axis=1)
+        a = int(np.argmin(d2))
+        if d2[a] >= 1e-10:
+            raise PkError("shell %d does not match any atom" % s)
+        out[a] += gshell[s]
+    return out
 
 
 class FockEngine:
@@ -230,6 +246,32 @@ class FockEngine:
         v = ctypes.c_void_p
         _check(load().pk_fock_jk_uhf(self.ctx, v(Pa_ptr), v(Pb_ptr), v(Jt_ptr), v(Ka_ptr), v(Kb_ptr), MEM_DEVICE),
                "pk_fock_jk_uhf")
+
+    def eri_gradient(self, Pj, Pk1=None, Pk2=None, kx=1.0):
+        """Two-electron gradient per shell centre (nshells x 3, host arrays):
+        dE2/dR with E2 = 1/4 sum Gamma_ijkl (ij|kl),
+        Gamma = 2 Pj_ij Pj_kl - kx (Pk1_ik Pk1_jl + Pk2_ik Pk2_jl) (pk_eri_gradient).
+        RHF: eri_gradient(P); UHF: eri_gradient(Pa + Pb, Pa, Pb, 2.0)."""
+        Pj = np.ascontiguousarray(Pj, dtype=np.float64)
+        Pk1 = Pj if Pk1 is None else np.ascontiguousarray(Pk1, dtype=np.float64)
+        Pk2 = None if Pk2 is None else np.ascontiguousarray(Pk2, dtype=np.float64)
+        for M in (Pk1, Pk2):
+            if M is not None and M.shape != Pj.shape:
+                raise ValueError("eri_gradient: densities of different shapes")
+        if self.nmol != 1:
+            raise ValueError("eri_gradient: one molecule per context")
+        if Pj.shape != (self.nbasis, self.nbasis):
+            raise ValueError("eri_gradient: density must be nbasis x nbasis")
+        ns = len(self._keep.l)
+        g = np.zeros((ns, 3))
+        _check(load().pk_eri_gradient(self.ctx, _ptr(Pj), _ptr(Pk1), None if Pk2 is None else _ptr(Pk2), float(kx),
+                                      _ptr(g), MEM_HOST), "pk_eri_gradient")
+        return g
+
+    def eri_gradient_atoms(self, xyz_bohr, Pj, Pk1=None, Pk2=None, kx=1.0):
+        """eri_gradient summed over the shells of each atom (natoms x 3): a
+        shell belongs to the atom at its centre (within 1e-5 Bohr)."""
+        return shells_to_atoms(self.eri_gradient(Pj, Pk1, Pk2, kx), self._keep.center, xyz_bohr)
 
     def g(self, P):
         P = np.ascontiguousarray(P, dtype=np.float64)

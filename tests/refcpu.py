@@ -50,6 +50,9 @@ def lib():
         L.pkref_rys_rule.argtypes = [ip, d, vp, vp]
         L.pkref_auto_prefers_rys.argtypes = [vp, vp, vp, vp]
         L.pkref_syev.argtypes = [ip, vp, vp, vp]
+        L.pkref_eri_deriv.argtypes = [vp, ip, ip, ip, ip, vp]
+        L.pkref_eri_gradient.argtypes = [vp, vp, vp, vp, d, vp, ip]
+        L.pkref_screen_primitives.argtypes = [vp, d, ip]
         _lib = L
     return _lib
 
@@ -148,6 +151,31 @@ class Ref:
         if lib().pkref_jk_elements(self.ctx, _p(P), len(r), _p(r), _p(c), _p(J), _p(K), nthreads) != 0:
             raise ValueError(lib().pkref_last_error().decode())
         return J, K
+
+    def screen_primitives(self, cut, rule=1):
+        """Drop primitive pairs as the engine does (rule 0: prefactor bound,
+        rule 1: primitive Schwarz bound); returns the number dropped."""
+        return lib().pkref_screen_primitives(self.ctx, float(cut), int(rule))
+
+    def eri_deriv(self, i, j, k, l):
+        """d(ij|kl)/dX_q for the centres of i, j, k, l (i <= j, k <= l): a 4 x 3 array."""
+        out = np.zeros(12)
+        if lib().pkref_eri_deriv(self.ctx, i, j, k, l, _p(out)) != 0:
+            raise ValueError(lib().pkref_last_error().decode())
+        return out.reshape(4, 3)
+
+    def eri_gradient(self, Pj, Pk1=None, Pk2=None, kx=1.0, nthreads=0):
+        """Two-electron gradient (natoms x 3): 1/4 sum Gamma d(ij|kl)/dR with
+        Gamma = 2 Pj_ij Pj_kl - kx (Pk1_ik Pk1_jl + Pk2_ik Pk2_jl)."""
+        Pj = np.ascontiguousarray(Pj, dtype=np.float64)
+        Pk1 = Pj if Pk1 is None else np.ascontiguousarray(Pk1, dtype=np.float64)
+        Pk2 = None if Pk2 is None else np.ascontiguousarray(Pk2, dtype=np.float64)
+        g = np.zeros((len(self.Z), 3))
+        rc = lib().pkref_eri_gradient(self.ctx, _p(Pj), _p(Pk1), None if Pk2 is None else _p(Pk2), float(kx),
+                                      _p(g), nthreads)
+        if rc != 0:
+            raise ValueError(lib().pkref_last_error().decode())
+        return g
 
     def rhf(self, charge=0, tol_energy=1e-10, tol_density=1e-10, diis_dim=8, max_cycles=0):
         E, it = ctypes.c_double(), ctypes.c_int()

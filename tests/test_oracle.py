@@ -12,6 +12,10 @@ What pins each part (none of these re-types the oracle's own formula):
   auto routing ...... the per-class Rys fractions of the reference cost model
   Rys rule .......... moment conditions sum_r w_r u_r^k = F_k(T), k < 2n (n <= 5;
                       the reference's moment method degrades from n = 7 on)
+  ERI derivatives ... central finite differences (Richardson-extrapolated) of the
+  and 2e gradient     oracle's own two-electron energy 1/2 Tr[Pj J] - kx/4 sum Tr[Pk K]
+                      at displaced geometries, fixed densities, accurate Boys;
+                      translation invariance of every derivative quartet
 """
 import json
 import math
@@ -373,3 +377,66 @@ def test_sad_oracle_invariants():
     Pa, Pb, Sh = sad_ref.atomic_uhf(h, 1)
     Hh = h.matrix("kinetic") + h.matrix("nuclear")
     assert abs((Pa * Hh).sum() - (-0.46658185)) < 1e-8
+
+
+def _e2(basis, Z, X, Pj, Pks, kx):
+    r = refcpu.Ref(basis, Z, X, engine="os", tol_eri=0.0)
+    J, _ = r.jk(Pj)
+    e = 0.5 * np.sum(Pj * J)
+    for Pk in Pks:
+        _, K = r.jk(Pk)
+        e -= 0.25 * kx * np.sum(Pk * K)
+    return e
+
+
+@pytest.mark.parametrize("basis", ["6-31g*", "cc-pvdz"])
+def test_eri_gradient_against_finite_differences(basis):
+    """The oracle's 2e gradient (gradient.cpp:209-232 with os.cpp:1296-1420)
+    is the derivative of the oracle's own 2e energy at fixed densities: RHF
+    form (Pj = Pk = P, kx = 1) and UHF form (Pj = Pa + Pb, Pk = Pa, Pb, kx = 2).
+    A dropped 2 alpha weight, a wrong sign of the lowering term, a missed
+    permutation or a wrong centre all move it by O(1)."""
+    Z, X = water()
+    X = X * ANGSTROM_TO_BOHR
+    ref = refcpu.Ref(basis, Z, X, engine="os", tol_eri=0.0)
+    n = ref.nbasis
+    rng = np.random.default_rng(1)
+    A = rng.normal(size=(n, n)) * 0.2
+    Pa = A + A.T
+    B = rng.normal(size=(n, n)) * 0.2
+    Pb = B + B.T
+    with refcpu.exact_boys():
+        g_rhf = ref.eri_gradient(Pa, Pa, None, 1.0)
+        g_uhf = ref.eri_gradient(Pa + Pb, Pa, Pb, 2.0)
+        for g, Pj, Pks, kx in ((g_rhf, Pa, [Pa], 1.0), (g_uhf, Pa + Pb, [Pa, Pb], 2.0)):
+            def d(a, q, h):
+                Xp, Xm = X.copy(), X.copy()
+                Xp[a, q] += h
+                Xm[a, q] -= h
+                return (_e2(basis, Z, Xp, Pj, Pks, kx) - _e2(basis, Z, Xm, Pj, Pks, kx)) / (2 * h)
+            fd = np.array([[(4 * d(a, q, 5e-4) - d(a, q, 1e-3)) / 3 for q in range(3)] for a in range(len(Z))])
+            err = np.abs(g - fd).max()
+            print("%s kx=%g: max|g - fd| = %.2e (|g| up to %.2f)" % (basis, kx, err, np.abs(g).max()))
+            assert err < 2e-9 * max(1.0, np.abs(g).max())
+            assert np.abs(g.sum(axis=0)).max() < 1e-11  # translation invariance
+
+
+def test_eri_derivative_translation_invariance():
+    """sum over the four centres of d(ij|kl)/dX_q vanishes for every quartet
+    (the integral depends on differences of centres only); d/dB of (ij|kl) is
+    d/dA of (ji|kl) up to the centre relabelling."""
+    Z, X = water()
+    ref = refcpu.Ref("6-31g*", Z, X * ANGSTROM_TO_BOHR, engine="os", tol_eri=0.0)
+    n = ref.nbasis
+    rng = np.random.default_rng(2)
+    worst = 0.0
+    for _ in range(200):
+        i, j, k, l = rng.integers(0, n, 4)
+        i, j = min(i, j), max(i, j)
+        k, l = min(k, l), max(k, l)
+        D = ref.eri_deriv(int(i), int(j), int(k), int(l))
+        worst = max(worst, np.abs(D.sum(axis=0)).max() / max(1e-3, np.abs(D).max()))
+        # bra-ket swap: d(kl|ij) on the centres of k, l, i, j
+        D2 = ref.eri_deriv(int(k), int(l), int(i), int(j))
+        assert np.allclose(D2[[2, 3, 0, 1]], D, rtol=1e-12, atol=1e-14)
+    assert worst < 1e-10
