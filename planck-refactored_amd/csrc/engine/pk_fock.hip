@@ -9,6 +9,7 @@
 // src/integrals/os.cpp:1435-1625 and rys.cpp:701-824 (the Fock builds),
 // os.cpp:1703-1752 (Schwarz table), shellpair.cpp:7-22 (pairs).
 #include "pk_fock.h"
+#include "pk_internal.h"
 
 #include <hip/hip_runtime.h>
 
@@ -664,6 +665,22 @@ int finish_timing(pk_ctx *c) {
 }
 
 }  // namespace
+
+int pk_build_with_scratch(pk_ctx *c, const double *P, double *J, double *K, double *scratch, size_t each,
+                          const char *caller) {
+    const std::string who = caller ? caller : "pk_build_with_scratch";
+    if (!c || !P || !scratch) return fail(PK_ERR_ARG, who + ": null context, density or scratch");
+    if (c->n2 > each)
+        return fail(PK_ERR_ARG, who + ": packed matrices of " + std::to_string(c->n2) +
+                                    " elements exceed the scratch limit of " + std::to_string(each) + " elements");
+    if (!c->work_ok)
+        return fail(PK_ERR_STATE, who + ": the work lists of the current settings are not built "
+                                        "(call pk_fock_jk once after pk_ctx_create / pk_ctx_set)");
+    const BuildBufs sb{scratch, scratch + each, scratch + 2 * each};
+    return launch_build(c, P, J, K, nullptr, sb, false);
+}
+
+int pk_set_error(int code, const char *msg) { return fail(code, msg ? msg : ""); }
 
 // ====================================================================== C ABI
 extern "C" {
