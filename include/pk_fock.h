@@ -131,6 +131,18 @@ int pk_fock_2e(pk_ctx *ctx, const double *P, double *G, int memkind);
 int pk_fock_jk_uhf(pk_ctx *ctx, const double *Pa, const double *Pb, double *Jt, double *Ka, double *Kb,
                    int memkind);
 
+/* pk_fock_jk without any device allocation in the call.  No counterpart in the
+ * reference: for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  P, J, K are device
+ * pointers; the build is enqueued on the context's stream and the call returns
+ * without synchronising.  The per-build scratch is a static device array of
+ * the library (24 MiB per GPU, held until the process ends): the packed
+ * matrices may hold at most 1048576 elements (one molecule of up to 1024
+ * functions; PK_ERR_ARG beyond), calls that overlap on one GPU race, and the
+ * work lists of the current settings must already exist (pk_fock_jk once after
+ * pk_ctx_create / pk_ctx_set; PK_ERR_STATE otherwise). */
+int pk_fock_jk_noalloc(pk_ctx *ctx, const double *P, double *J, double *K);
+
 /* Q_ij = sqrt|(ij|ij)|, nb x nb (the reference's Schwarz table). */
 int pk_schwarz(pk_ctx *ctx, double *Q, int memkind);
 /* Full screened nb^4 ERI tensor, index ((i*nb + j)*nb + k)*nb + l, as the
