@@ -5,6 +5,13 @@ n-C10H22 / cc-pVTZ, 680 functions), Schwarz 1e-12, engine os.
 The oracle's full build at these sizes takes hours on the CPU; its element
 sampler (pkref_jk_elements) sums every AO quartet of one element on its own
 with the screening of the full build, so a sample of elements costs minutes.
+
+Two sets of values per density: J_<kind>, K_<kind> with the oracle's accurate
+Boys function (pkref_set_boys_exact, ~1e-17 relative; the bar of the tests),
+and J_<kind>_lookup, K_<kind>_lookup with the reference's own table lookup
+(src/lookup/boys.cpp:1592-1618, ~1e-12 absolute), whose error the high-L
+integrals of cc-pVTZ amplify to a few 1e-8 in some d/f elements; the tests
+report that difference beside their check.
 Two densities per molecule, regenerated from their seeds by the tests:
   signed:   P = 0.05 (A + A^T),        A ~ N(0, 1)  (numpy default_rng(seed))
   positive: P = 0.05 (|A| + |A|^T)     (every product of P adds up with one sign,
@@ -27,7 +34,7 @@ sys.path.insert(0, os.path.join(ROOT, "planck-refactored_amd"))
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 from pkfock.molecules import ANGSTROM_TO_BOHR, alkane  # noqa: E402
-from refcpu import Ref  # noqa: E402
+from refcpu import Ref, exact_boys  # noqa: E402
 
 WORKLOADS = {"c20h42_ccpvdz": (20, "cc-pvdz"), "c10h22_ccpvtz": (10, "cc-pvtz")}
 SEEDS = {"signed": 101, "positive": 102}
@@ -75,12 +82,16 @@ def main(names):
         rows, cols = elements(fl)
         out = {"rows": rows, "cols": cols}
         meta = {"workload": name, "basis": basis, "carbons": ncarb, "nbasis": n, "tol_eri": 1e-12,
-                "engine": "os", "seeds": SEEDS, "scale": 0.05}
+                "engine": "os", "seeds": SEEDS, "scale": 0.05, "boys": "accurate; *_lookup: reference lookup"}
         for kind in SEEDS:
             P = density(n, kind)
             meta["P_checksum_" + kind] = [float(P.sum()), float((P * P).sum())]
             t = time.perf_counter()
             J, K = ref.jk_elements(P, rows, cols)
+            out["J_%s_lookup" % kind] = J
+            out["K_%s_lookup" % kind] = K
+            with exact_boys():
+                J, K = ref.jk_elements(P, rows, cols)
             print("%s %s: %d elements in %.1f s" % (name, kind, len(rows), time.perf_counter() - t), flush=True)
             out["J_" + kind] = J
             out["K_" + kind] = K

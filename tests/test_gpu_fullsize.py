@@ -4,14 +4,19 @@ n-C20H42 / cc-pVDZ (510 functions) and n-C10H22 / cc-pVTZ (680 functions,
 f shells): the engine's J and K (the path of src/integrals/os.cpp:1435-1523)
 against oracle values committed in tests/golden/jk_elements_*.npz (written by
 tests/golden/make_fullsize_jk.py: every AO quartet of each sampled element
-summed on the CPU, no primitive screening, reference Boys lookup).  Two seeded
+summed on the CPU, no primitive screening).  The bar applies to the oracle with
+its accurate Boys function; the same elements with the reference's Boys table
+lookup (~1e-12 absolute in F_m, amplified by the high-L d/f integrals of
+cc-pVTZ) differ from the accurate values by up to 4e-8, and that difference is
+printed beside each check.  Two seeded
 densities each; one of them positive everywhere, so that small terms the
 engine drops (primitive pairs below its prefactor bound) add up with one sign.
 
 Bars: FP64 J/K within 1e-9 absolute (the Fock bar).  Mixed precision: every
-integral of a quartet below the cutoff c is computed in FP32 with a relative
-error well below 1e-5, and such an integral is below c in magnitude, so an
-element moves by less than 1e-5 c sum_kl |P_kl|; the test uses that bound.
+integral of a quartet whose Schwarz bound lies below the cutoff c is computed
+in FP32 within 2e-5 of that bound (tests/test_gpu_eri_templates.py measures
+1e-5 at most), so an element moves by less than 2e-5 c sum_kl |P_kl|; the test
+uses that bound.
 """
 import json
 import os
@@ -73,8 +78,11 @@ def test_fullsize_fp64_jk_matches_oracle(name):
         J, K = eng.jk(P)
         dj = np.abs(J[r, c] - d["J_" + kind]).max()
         dk = np.abs(K[r, c] - d["K_" + kind]).max()
-        print("%s %s: %d elements  max|dJ| %.2e  max|dK| %.2e  (|J| up to %.1f, |K| up to %.1f)" % (
-            name, kind, len(r), dj, dk, np.abs(d["J_" + kind]).max(), np.abs(d["K_" + kind]).max()))
+        lj = np.abs(J[r, c] - d["J_%s_lookup" % kind]).max()
+        lk = np.abs(K[r, c] - d["K_%s_lookup" % kind]).max()
+        print("%s %s: %d elements  max|dJ| %.2e  max|dK| %.2e  (|J| up to %.1f, |K| up to %.1f); "
+              "vs the reference-lookup oracle %.2e / %.2e" % (
+                  name, kind, len(r), dj, dk, np.abs(d["J_" + kind]).max(), np.abs(d["K_" + kind]).max(), lj, lk))
         assert dj < FOCK_TOL and dk < FOCK_TOL
         assert np.array_equal(J, J.T) and np.array_equal(K, K.T)
 
@@ -91,7 +99,7 @@ def test_fullsize_mixed_jk_matches_oracle(name, cutoff):
             J, K = eng.jk(P)
             dj = np.abs(J[r, c] - d["J_" + kind]).max()
             dk = np.abs(K[r, c] - d["K_" + kind]).max()
-            bar = max(FOCK_TOL, 1e-5 * cutoff * np.abs(P).sum(axis=1).max())
+            bar = max(FOCK_TOL, 2e-5 * cutoff * np.abs(P).sum(axis=1).max())
             n32 = sum(s["quartets"] for s in eng.class_stats() if s["precision"])
             print("%s mixed %g %s: FP32 quartets %.1f%%  max|dJ| %.2e  max|dK| %.2e  bar %.1e" % (
                 name, cutoff, kind, 100.0 * n32 / eng.stats()["quartets"], dj, dk, bar))
