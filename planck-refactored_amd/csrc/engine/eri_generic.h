@@ -112,6 +112,9 @@ struct GenTabs {
 __global__ __launch_bounds__(64) void k_generic(GenArgs a) {
     extern __shared__ double smem[];
     __shared__ double su[GEN_NR], sw[GEN_NR];
+    __shared__ double hab[3][10], hcd[3][10];  // C(b,k) AB^(b-k) per axis at [b(b+1)/2 + k]; ket likewise
+    __shared__ double sP[6][100];              // density blocks ab, cd, ac, ad, bc, bd of the quartet
+    __shared__ double sQ[2][100];              // Schwarz blocks ab, cd
     __shared__ GenTabs tb;
     const int lane = threadIdx.x;
     const int item = blockIdx.x;
@@ -154,8 +157,10 @@ __global__ __launch_bounds__(64) void k_generic(GenArgs a) {
         tb.g_a[g] = t / D2;
     }
     // this lane's first (ab, cd) in the flattened component loops, and the
-    // step of 64 components in (ab, cd)
+    // step of 64 components in (ab, cd); likewise (root-axis, g) of the
+    // transfer loop
     const int ab0 = lane / NCD, cd0 = lane % NCD, sab = 64 / NCD, scd = 64 % NCD;
+    const int ra0 = lane / NG, g0 = lane % NG, sra = 64 / NG, sg = 64 % NG;
     __syncthreads();
 
     int bra, kbeg, kcnt, nb = 0;
@@ -178,11 +183,28 @@ __global__ __launch_bounds__(64) void k_generic(GenArgs a) {
         Bm = a.B + moff;
     }
     const PairDev pb = a.pairs[bra];
+    if (lane < 30) {
+        // bra transfer coefficients C(b,k) AB^(b-k), b <= 3
+        const int ax = lane / 10, t = lane % 10;
+        const int b = t >= 6 ? 3 : (t >= 3 ? 2 : (t >= 1 ? 1 : 0)), k = t - b * (b + 1) / 2;
+        const double x = sel3(ax, pb.ab[0], pb.ab[1], pb.ab[2]);
+        double p = binom(b, k);
+        for (int e = 0; e < b - k; ++e) p *= x;
+        hab[ax][t] = p;
+    }
 
     for (int kk = 0; kk < kcnt; ++kk) {
         const int ket = a.eri_mode ? a.qket[item] : a.kets[kbeg + kk];
         const PairDev pk = a.pairs[ket];
         for (int i = lane; i < NABCD; i += 64) acc[i] = 0.0;
+        if (lane < 30) {
+            const int ax = lane / 10, t = lane % 10;
+            const int b = t >= 6 ? 3 : (t >= 3 ? 2 : (t >= 1 ? 1 : 0)), k = t - b * (b + 1) / 2;
+            const double x = sel3(ax, pk.ab[0], pk.ab[1], pk.ab[2]);
+            double p = binom(b, k);
+            for (int e = 0; e < b - k; ++e) p *= x;
+            hcd[ax][t] = p;
+        }
         __syncthreads();
         for (int kq = 0; kq < pk.nprim; ++kq) {
             const PrimDev q = a.prims[pk.prim_off + kq];
@@ -244,25 +266,25 @@ __global__ __launch_bounds__(64) void k_generic(GenArgs a) {
                 }
                 __syncthreads();
                 // transferred factors G[a][b][c][d] = sum_k sum_l C(b,k) AB^(b-k)
-                // C(d,l) CD^(d-l) I[a+k][c+l]; (root, axis) uniform, G index over lanes
-                for (int ra = 0; ra < 3 * n; ++ra) {
-                    const int ax = ra % 3;
-                    const double ABq = sel3(ax, pb.ab[0], pb.ab[1], pb.ab[2]);
-                    const double CDq = sel3(ax, pk.ab[0], pk.ab[1], pk.ab[2]);
-                    const double *I = I2 + ra * NI;
-                    for (int g = lane; g < NG; g += 64) {
+                // C(d,l) CD^(d-l) I[a+k][c+l] for every (root, axis) and G index,
+                // spread over the lanes
+                {
+                    int ra = ra0, g = g0;
+                    for (int t = lane; t < 3 * n * NG; t += 64) {
+                        const int ax = ra % 3;
                         const int ia = tb.g_a[g], ib = tb.g_b[g], ic = tb.g_c[g], id = tb.g_d[g];
-                        double s = 0.0, pab = 1.0;
-                        for (int k = ib; k >= 0; --k) {
-                            double scd = 0.0, pcd = 1.0;
-                            for (int l = id; l >= 0; --l) {
-                                scd = fma(binom(id, l) * pcd, I[(ia + k) * F1 + ic + l], scd);
-                                pcd *= CDq;
-                            }
-                            s = fma(binom(ib, k) * pab, scd, s);
-                            pab *= ABq;
+                        const double *I = I2 + ra * NI + ia * F1 + ic;
+                        const double *hb = hab[ax] + ib * (ib + 1) / 2, *hd = hcd[ax] + id * (id + 1) / 2;
+                        double s = 0.0;
+                        for (int k = 0; k <= ib; ++k) {
+                            double scd = 0.0;
+                            for (int l = 0; l <= id; ++l) scd = fma(hd[l], I[k * F1 + l], scd);
+                            s = fma(hb[k], scd, s);
                         }
                         G[ra * NG + g] = s;
+                        g += sg;
+                        ra += sra;
+                        if (g >= NG) g -= NG, ++ra;
                     }
                 }
                 __syncthreads();
@@ -295,16 +317,30 @@ __global__ __launch_bounds__(64) void k_generic(GenArgs a) {
         if (pk.diag) deg *= 0.5;
         if (bra == ket) deg *= 0.5;
         const bool all_pass = pb.qmin * pk.qmin >= a.tol;
+        // density (and Schwarz) blocks of the quartet into LDS: independent
+        // loads issued together, read many times by the reductions below
+        auto Pv = [&](int r, int c) { return P[(size_t)r * nb + c]; };
+        for (int i = lane; i < NAB; i += 64) {
+            const int af = pb.f0 + tb.ab_a[i], bf = pb.f1 + tb.ab_b[i];
+            sP[0][i] = Pv(af, bf);
+            if (!all_pass) sQ[0][i] = Qm[(size_t)af * nb + bf];
+        }
+        for (int i = lane; i < NCD; i += 64) {
+            const int cf = pk.f0 + tb.cd_c[i], df = pk.f1 + tb.cd_d[i];
+            sP[1][i] = Pv(cf, df);
+            if (!all_pass) sQ[1][i] = Qm[(size_t)cf * nb + df];
+        }
+        for (int i = lane; i < NA * NC; i += 64) sP[2][i] = Pv(pb.f0 + tb.ac_a[i], pk.f0 + tb.ac_c[i]);
+        for (int i = lane; i < NA * ND; i += 64) sP[3][i] = Pv(pb.f0 + tb.ad_a[i], pk.f1 + tb.ad_d[i]);
+        for (int i = lane; i < NB * NC; i += 64) sP[4][i] = Pv(pb.f1 + tb.bc_b[i], pk.f0 + tb.bc_c[i]);
+        for (int i = lane; i < NB * ND; i += 64) sP[5][i] = Pv(pb.f1 + tb.bd_b[i], pk.f1 + tb.bd_d[i]);
+        __syncthreads();
         // apply degeneracy and the function-level Schwarz test in place
         {
             int ab = ab0, cd = cd0;
             for (int i = lane; i < NABCD; i += 64) {
                 double m = deg;
-                if (!all_pass) {
-                    const int af = pb.f0 + tb.ab_a[ab], bf = pb.f1 + tb.ab_b[ab];
-                    const int cf = pk.f0 + tb.cd_c[cd], df = pk.f1 + tb.cd_d[cd];
-                    if (Qm[(size_t)af * nb + bf] * Qm[(size_t)cf * nb + df] < a.tol) m = 0.0;
-                }
+                if (!all_pass && sQ[0][ab] * sQ[1][cd] < a.tol) m = 0.0;
                 acc[i] *= m;
                 cd += scd;
                 ab += sab;
@@ -312,31 +348,28 @@ __global__ __launch_bounds__(64) void k_generic(GenArgs a) {
             }
         }
         __syncthreads();
-        auto Pv = [&](int r, int c) { return P[(size_t)r * nb + c]; };
         auto at = [&](int ia, int ib, int ic, int id) { return acc[(ia * NB + ib) * NCD + ic * ND + id]; };
         // A_ab += 2 sum_cd v P_cd ; A_cd += 2 sum_ab v P_ab
-        seg_reduce(lane, NAB, NCD, [&](int ab, int cd) {
-            return acc[ab * NCD + cd] * Pv(pk.f0 + tb.cd_c[cd], pk.f1 + tb.cd_d[cd]);
-        }, [&](int ab, double v) { unsafeAtomicAdd(&Am[(size_t)(pb.f0 + tb.ab_a[ab]) * nb + pb.f1 + tb.ab_b[ab]], 2.0 * v); });
-        seg_reduce(lane, NCD, NAB, [&](int cd, int ab) {
-            return acc[ab * NCD + cd] * Pv(pb.f0 + tb.ab_a[ab], pb.f1 + tb.ab_b[ab]);
-        }, [&](int cd, double v) { unsafeAtomicAdd(&Am[(size_t)(pk.f0 + tb.cd_c[cd]) * nb + pk.f1 + tb.cd_d[cd]], 2.0 * v); });
+        seg_reduce(lane, NAB, NCD, [&](int ab, int cd) { return acc[ab * NCD + cd] * sP[1][cd]; },
+                   [&](int ab, double v) {
+                       unsafeAtomicAdd(&Am[(size_t)(pb.f0 + tb.ab_a[ab]) * nb + pb.f1 + tb.ab_b[ab]], 2.0 * v);
+                   });
+        seg_reduce(lane, NCD, NAB, [&](int cd, int ab) { return acc[ab * NCD + cd] * sP[0][ab]; },
+                   [&](int cd, double v) {
+                       unsafeAtomicAdd(&Am[(size_t)(pk.f0 + tb.cd_c[cd]) * nb + pk.f1 + tb.cd_d[cd]], 2.0 * v);
+                   });
         // B_ac += sum_bd v P_bd ; B_bc += sum_ad v P_ad ; B_ad += sum_bc v P_bc ; B_bd += sum_ac v P_ac
         seg_reduce(lane, NA * NC, NB * ND, [&](int o, int t) {
-            const int ib = tb.bd_b[t], id = tb.bd_d[t];
-            return at(tb.ac_a[o], ib, tb.ac_c[o], id) * Pv(pb.f1 + ib, pk.f1 + id);
+            return at(tb.ac_a[o], tb.bd_b[t], tb.ac_c[o], tb.bd_d[t]) * sP[5][t];
         }, [&](int o, double v) { unsafeAtomicAdd(&Bm[(size_t)(pb.f0 + tb.ac_a[o]) * nb + pk.f0 + tb.ac_c[o]], v); });
         seg_reduce(lane, NB * NC, NA * ND, [&](int o, int t) {
-            const int ia = tb.ad_a[t], id = tb.ad_d[t];
-            return at(ia, tb.bc_b[o], tb.bc_c[o], id) * Pv(pb.f0 + ia, pk.f1 + id);
+            return at(tb.ad_a[t], tb.bc_b[o], tb.bc_c[o], tb.ad_d[t]) * sP[3][t];
         }, [&](int o, double v) { unsafeAtomicAdd(&Bm[(size_t)(pb.f1 + tb.bc_b[o]) * nb + pk.f0 + tb.bc_c[o]], v); });
         seg_reduce(lane, NA * ND, NB * NC, [&](int o, int t) {
-            const int ib = tb.bc_b[t], ic = tb.bc_c[t];
-            return at(tb.ad_a[o], ib, ic, tb.ad_d[o]) * Pv(pb.f1 + ib, pk.f0 + ic);
+            return at(tb.ad_a[o], tb.bc_b[t], tb.bc_c[t], tb.ad_d[o]) * sP[4][t];
         }, [&](int o, double v) { unsafeAtomicAdd(&Bm[(size_t)(pb.f0 + tb.ad_a[o]) * nb + pk.f1 + tb.ad_d[o]], v); });
         seg_reduce(lane, NB * ND, NA * NC, [&](int o, int t) {
-            const int ia = tb.ac_a[t], ic = tb.ac_c[t];
-            return at(ia, tb.bd_b[o], ic, tb.bd_d[o]) * Pv(pb.f0 + ia, pk.f0 + ic);
+            return at(tb.ac_a[t], tb.bd_b[o], tb.ac_c[t], tb.bd_d[o]) * sP[2][t];
         }, [&](int o, double v) { unsafeAtomicAdd(&Bm[(size_t)(pb.f1 + tb.bd_b[o]) * nb + pk.f1 + tb.bd_d[o]], v); });
         __syncthreads();
     }
