@@ -10,7 +10,8 @@ build from a device-resident density (seeded random symmetric matrix) to
 device-resident J and K.
 
 Measured, in this order:
-  1. the plain FP64 build: W untimed + K timed steps;
+  1. the plain FP64 build: W untimed + K timed steps (default kernels and
+     launch shapes; --autotune lets the engine's autotuner pick them first);
   2. the mixed-precision build (quartets whose Schwarz bound lies below the
      FP32 cutoff computed in FP32, everything else FP64) at each cutoff of
      --cutoffs (default 1e-4 .. 1e-7), W + K steps each;
@@ -54,8 +55,9 @@ WORKLOADS = {
 def print_classes(cls):
     """Per-class device time of one profiled build, slowest first (stderr)."""
     for c in sorted(cls, key=lambda c: -c["ms_avg"]):
-        print("  class (%d%d|%d%d)%s x%d kernels: %8.3f ms  %.3e quartets  %.2f TFLOP/s  %.1f kets/wave" % (
-            c["la"], c["lb"], c["lc"], c["ld"], " fp32" if c["precision"] else "", c["kernels"], c["ms_avg"],
+        print("  class (%d%d|%d%d)%s%s x%d kernels: %8.3f ms  %.3e quartets  %.2f TFLOP/s  %.1f kets/wave" % (
+            c["la"], c["lb"], c["lc"], c["ld"], " fp32" if c["precision"] else "", " alt" if c.get("alt") else "",
+            c["kernels"], c["ms_avg"],
             c["quartets"], c["model_flops"] / (c["ms_avg"] * 1e-3) * 1e-12,
             c["quartets"] / max(c.get("waves", 0), 1)), file=sys.stderr, flush=True)
 
@@ -272,6 +274,9 @@ def bench_alkane(args):
             eng.set("os", 1e-12)
         else:
             eng.set("os", 1e-12, precision=PREC_MIXED, fp32_cutoff=cut)
+        if args.autotune:
+            eng.jk_device(P.data_ptr(), J.data_ptr(), K.data_ptr())  # work lists of this configuration
+            eng.autotune(reps=2)
         ms, elapsed, cls = timed_builds(eng, P, J, K, args.steps, args.warmup)
         st = eng.stats()
         dom = max(cls, key=lambda c: c["ms_avg"])
@@ -384,6 +389,9 @@ def main():
     ap.add_argument("--no-seq", action="store_true", help="water clusters: skip the one-at-a-time comparisons")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--verbose", action="store_true", help="per-class times of every configuration")
+    ap.add_argument("--autotune", action="store_true",
+                    help="let the engine's autotuner pick each class list's kernel and launch shape per configuration "
+                         "(its trial builds then also appear in a kernel trace of the run)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="c10h22-ccpvtz")
     ap.add_argument("--cutoffs", default="1e-4,1e-5,1e-6,1e-7",
@@ -417,7 +425,12 @@ def main():
 
 
 def dom_name(c):
-    return "k_jk<%s,%d,%d,%d,%d>" % ("float" if c["precision"] else "double", c["la"], c["lb"], c["lc"], c["ld"])
+    """Kernel that runs a class list (pk_class_stats.kind)."""
+    cls = "%d,%d,%d,%d" % (c["la"], c["lb"], c["lc"], c["ld"])
+    kind = c.get("kind", 0)
+    if kind == 2:
+        return "k_generic (%s)" % cls
+    return "k_jk<%s,%s,%s>" % ("float" if kind >= 3 else "double", cls, "OS" if kind in (0, 3) else "Rys")
 
 
 if __name__ == "__main__":

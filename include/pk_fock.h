@@ -106,7 +106,9 @@ int pk_ctx_mol_layout(const pk_ctx *ctx, int32_t *nb, int64_t *offset);
 int pk_ctx_set_graph(pk_ctx *ctx, int enable);
 
 /* Launch-shape autotuner: times every class list under each candidate shape
- * (1, 2 or 4 waves per workgroup; with or without XCD-contiguous wave ranges)
+ * (1, 2 or 4 waves per workgroup; with or without XCD-contiguous wave ranges;
+ * the list's alternative kernel: FP64 integrals for an FP32 list of the mixed
+ * precision, the cooperative kernel for a class with per-lane Rys kernels)
  * over `reps` builds and keeps the fastest per class for this context.  The
  * density buffer is overwritten. */
 int pk_ctx_autotune(pk_ctx *ctx, int reps);
@@ -125,9 +127,10 @@ int pk_fock_jk(pk_ctx *ctx, const double *P, double *J, double *K, int memkind);
 /* G = J[P] - K[P]/2, the return value of _compute_2e_fock (base.h:57-73). */
 int pk_fock_2e(pk_ctx *ctx, const double *P, double *G, int memkind);
 /* UHF: Jt = J[Pa + Pb], Ka = K[Pa], Kb = K[Pb]; G_a = Jt - Ka, G_b = Jt - Kb
- * (_compute_2e_fock_uhf, base.h:75-93).  Two builds (one per spin density)
- * and J summed on the device; host or device pointers; Jt, Ka or Kb may be
- * NULL.  Packed arrays for batches, as pk_fock_jk. */
+ * (_compute_2e_fock_uhf, base.h:75-93).  One pass over the integrals: each
+ * quartet is computed once (FP64) and digested into J of Pa + Pb and the two
+ * exchange matrices; host or device pointers; Jt, Ka or Kb may be NULL.
+ * Packed arrays for batches, as pk_fock_jk.  Runs eagerly (no graph). */
 int pk_fock_jk_uhf(pk_ctx *ctx, const double *Pa, const double *Pb, double *Jt, double *Ka, double *Kb,
                    int memkind);
 
@@ -136,7 +139,7 @@ int pk_fock_jk_uhf(pk_ctx *ctx, const double *Pa, const double *Pb, double *Jt, 
  * tight serving loops, harnesses that forbid hipMalloc).  P, J, K are device
  * pointers; the build is enqueued on the context's stream and the call returns
  * without synchronising.  The per-build scratch is a static device array of
- * the library (24 MiB per GPU, held until the process ends): the packed
+ * the library (28 MiB per GPU, held until the process ends): the packed
  * matrices may hold at most 1048576 elements (one molecule of up to 1024
  * functions; PK_ERR_ARG beyond), calls that overlap on one GPU race, and the
  * work lists of the current settings must already exist (pk_fock_jk once after
@@ -179,6 +182,12 @@ typedef struct pk_class_stats {
     int32_t block;                /* lanes per workgroup of the class kernels */
     int32_t remap;                /* 1: XCD-contiguous wave ranges */
     int64_t waves;                /* work items (one bra, up to 64 kets per wave) */
+    int32_t alt;                  /* 1: the list runs its alternative kernel (autotuned): FP64
+                                     integrals for an FP32 list, or the cooperative kernel for a
+                                     class with per-lane Rys kernels */
+    int32_t kind;                 /* kernels that run the list: 0 per-lane Obara-Saika (FP64),
+                                     1 per-lane Rys (FP64), 2 cooperative (FP64), 3 per-lane
+                                     Obara-Saika (FP32), 4 per-lane Rys (FP32) */
 } pk_class_stats;
 int pk_ctx_profile(pk_ctx *ctx, int enable);
 int pk_ctx_class_count(const pk_ctx *ctx);

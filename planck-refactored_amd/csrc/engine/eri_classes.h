@@ -70,7 +70,10 @@
 
 namespace pk {
 
-constexpr int kAccBudget = 48;  // contracted Rys accumulators per lane
+#ifndef PK_ACC_BUDGET
+#define PK_ACC_BUDGET 48
+#endif
+constexpr int kAccBudget = PK_ACC_BUDGET;  // contracted Rys accumulators per lane
 constexpr int kMaxChunks = 12;  // larger classes run in the cooperative kernel
 
 constexpr int chunk_size(int la, int lb, int lc, int ld) {
@@ -105,11 +108,11 @@ void launch_jk_rys(const JKArgs &a, int grid, int block, hipStream_t s, int &nla
     }
 }
 
-template <typename R, int LA, int LB, int LC, int LD>
+template <typename R, int LA, int LB, int LC, int LD, int NK = 1>
 void launch_jk_os(const JKArgs &a, int grid, int block, hipStream_t s, int &nlaunch) {
     if constexpr (os_available(LA, LB, LC, LD)) {
         constexpr int NCD = ncart(LC) * ncart(LD);
-        hipLaunchKernelGGL((k_jk<R, LA, LB, LC, LD, ALG_OS, 0, NCD>), dim3(grid), dim3(block), 0, s, a);
+        hipLaunchKernelGGL((k_jk<R, LA, LB, LC, LD, ALG_OS, 0, NCD, NK>), dim3(grid), dim3(block), 0, s, a);
         ++nlaunch;
     }
 }

@@ -35,10 +35,13 @@ struct GenArgs {
     const int *__restrict__ qket;
     const PairDev *__restrict__ pairs;
     const PrimDev *__restrict__ prims;
-    const double *__restrict__ P;
+    const double *__restrict__ P;    // density of J, scaled
+    const double *__restrict__ PK;   // exchange density of B (RHF: P)
+    const double *__restrict__ PK2;  // exchange density of B2 (k_generic<2>, one-pass UHF)
     const double *__restrict__ Q;
     double *A;
     double *B;
+    double *B2;
     double *out;  // ERI mode: [nitems][NA NB NC ND]
     double tol;
     int nitems;  // waves (JK) or quartets (ERI)
@@ -109,11 +112,16 @@ struct GenTabs {
     unsigned char g_a[256], g_b[256], g_c[256], g_d[256];      // G index -> exponents
 };
 
+// NK = 2: one-pass UHF digestion (J of P into A, exchange of PK into B and of
+// PK2 into B2).
+template <int NK>
 __global__ __launch_bounds__(64) void k_generic(GenArgs a) {
     extern __shared__ double smem[];
     __shared__ double su[GEN_NR], sw[GEN_NR];
     __shared__ double hab[3][10], hcd[3][10];  // C(b,k) AB^(b-k) per axis at [b(b+1)/2 + k]; ket likewise
-    __shared__ double sP[6][100];              // density blocks ab, cd, ac, ad, bc, bd of the quartet
+    // density blocks of the quartet: ab, cd (J density), ac, ad, bc, bd (exchange
+    // density of B), then ac, ad, bc, bd of the exchange density of B2
+    __shared__ double sP[NK == 2 ? 10 : 6][100];
     __shared__ double sQ[2][100];              // Schwarz blocks ab, cd
     __shared__ GenTabs tb;
     const int lane = threadIdx.x;
@@ -164,8 +172,8 @@ __global__ __launch_bounds__(64) void k_generic(GenArgs a) {
     __syncthreads();
 
     int bra, kbeg, kcnt, nb = 0;
-    const double *P = nullptr, *Qm = nullptr;
-    double *Am = nullptr, *Bm = nullptr;
+    const double *P = nullptr, *PKm = nullptr, *PK2m = nullptr, *Qm = nullptr;
+    double *Am = nullptr, *Bm = nullptr, *B2m = nullptr;
     if (a.eri_mode) {
         bra = a.qbra[item];
         kbeg = item;
@@ -178,9 +186,14 @@ __global__ __launch_bounds__(64) void k_generic(GenArgs a) {
         const long long moff = a.mol_off[mol];
         nb = a.mol_nb[mol];
         P = a.P + moff;
+        PKm = a.PK + moff;
         Qm = a.Q + moff;
         Am = a.A + moff;
         Bm = a.B + moff;
+        if (NK == 2) {
+            PK2m = a.PK2 + moff;
+            B2m = a.B2 + moff;
+        }
     }
     const PairDev pb = a.pairs[bra];
     if (lane < 30) {
@@ -330,10 +343,15 @@ __global__ __launch_bounds__(64) void k_generic(GenArgs a) {
             sP[1][i] = Pv(cf, df);
             if (!all_pass) sQ[1][i] = Qm[(size_t)cf * nb + df];
         }
-        for (int i = lane; i < NA * NC; i += 64) sP[2][i] = Pv(pb.f0 + tb.ac_a[i], pk.f0 + tb.ac_c[i]);
-        for (int i = lane; i < NA * ND; i += 64) sP[3][i] = Pv(pb.f0 + tb.ad_a[i], pk.f1 + tb.ad_d[i]);
-        for (int i = lane; i < NB * NC; i += 64) sP[4][i] = Pv(pb.f1 + tb.bc_b[i], pk.f0 + tb.bc_c[i]);
-        for (int i = lane; i < NB * ND; i += 64) sP[5][i] = Pv(pb.f1 + tb.bd_b[i], pk.f1 + tb.bd_d[i]);
+#pragma unroll
+        for (int sp = 0; sp < NK; ++sp) {
+            const double *X = sp == 0 ? PKm : PK2m;
+            double(*S)[100] = sP + 2 + 4 * sp;
+            for (int i = lane; i < NA * NC; i += 64) S[0][i] = X[(size_t)(pb.f0 + tb.ac_a[i]) * nb + pk.f0 + tb.ac_c[i]];
+            for (int i = lane; i < NA * ND; i += 64) S[1][i] = X[(size_t)(pb.f0 + tb.ad_a[i]) * nb + pk.f1 + tb.ad_d[i]];
+            for (int i = lane; i < NB * NC; i += 64) S[2][i] = X[(size_t)(pb.f1 + tb.bc_b[i]) * nb + pk.f0 + tb.bc_c[i]];
+            for (int i = lane; i < NB * ND; i += 64) S[3][i] = X[(size_t)(pb.f1 + tb.bd_b[i]) * nb + pk.f1 + tb.bd_d[i]];
+        }
         __syncthreads();
         // apply degeneracy and the function-level Schwarz test in place
         {
@@ -359,18 +377,24 @@ __global__ __launch_bounds__(64) void k_generic(GenArgs a) {
                        unsafeAtomicAdd(&Am[(size_t)(pk.f0 + tb.cd_c[cd]) * nb + pk.f1 + tb.cd_d[cd]], 2.0 * v);
                    });
         // B_ac += sum_bd v P_bd ; B_bc += sum_ad v P_ad ; B_ad += sum_bc v P_bc ; B_bd += sum_ac v P_ac
-        seg_reduce(lane, NA * NC, NB * ND, [&](int o, int t) {
-            return at(tb.ac_a[o], tb.bd_b[t], tb.ac_c[o], tb.bd_d[t]) * sP[5][t];
-        }, [&](int o, double v) { unsafeAtomicAdd(&Bm[(size_t)(pb.f0 + tb.ac_a[o]) * nb + pk.f0 + tb.ac_c[o]], v); });
-        seg_reduce(lane, NB * NC, NA * ND, [&](int o, int t) {
-            return at(tb.ad_a[t], tb.bc_b[o], tb.bc_c[o], tb.ad_d[t]) * sP[3][t];
-        }, [&](int o, double v) { unsafeAtomicAdd(&Bm[(size_t)(pb.f1 + tb.bc_b[o]) * nb + pk.f0 + tb.bc_c[o]], v); });
-        seg_reduce(lane, NA * ND, NB * NC, [&](int o, int t) {
-            return at(tb.ad_a[o], tb.bc_b[t], tb.bc_c[t], tb.ad_d[o]) * sP[4][t];
-        }, [&](int o, double v) { unsafeAtomicAdd(&Bm[(size_t)(pb.f0 + tb.ad_a[o]) * nb + pk.f1 + tb.ad_d[o]], v); });
-        seg_reduce(lane, NB * ND, NA * NC, [&](int o, int t) {
-            return at(tb.ac_a[t], tb.bd_b[o], tb.ac_c[t], tb.bd_d[o]) * sP[2][t];
-        }, [&](int o, double v) { unsafeAtomicAdd(&Bm[(size_t)(pb.f1 + tb.bd_b[o]) * nb + pk.f1 + tb.bd_d[o]], v); });
+        // (and the same into B2 with the second exchange density)
+#pragma unroll
+        for (int sp = 0; sp < NK; ++sp) {
+            double *Bs = sp == 0 ? Bm : B2m;
+            const double(*S)[100] = sP + 2 + 4 * sp;  // ac, ad, bc, bd
+            seg_reduce(lane, NA * NC, NB * ND, [&](int o, int t) {
+                return at(tb.ac_a[o], tb.bd_b[t], tb.ac_c[o], tb.bd_d[t]) * S[3][t];
+            }, [&](int o, double v) { unsafeAtomicAdd(&Bs[(size_t)(pb.f0 + tb.ac_a[o]) * nb + pk.f0 + tb.ac_c[o]], v); });
+            seg_reduce(lane, NB * NC, NA * ND, [&](int o, int t) {
+                return at(tb.ad_a[t], tb.bc_b[o], tb.bc_c[o], tb.ad_d[t]) * S[1][t];
+            }, [&](int o, double v) { unsafeAtomicAdd(&Bs[(size_t)(pb.f1 + tb.bc_b[o]) * nb + pk.f0 + tb.bc_c[o]], v); });
+            seg_reduce(lane, NA * ND, NB * NC, [&](int o, int t) {
+                return at(tb.ad_a[o], tb.bc_b[t], tb.bc_c[t], tb.ad_d[o]) * S[2][t];
+            }, [&](int o, double v) { unsafeAtomicAdd(&Bs[(size_t)(pb.f0 + tb.ad_a[o]) * nb + pk.f1 + tb.ad_d[o]], v); });
+            seg_reduce(lane, NB * ND, NA * NC, [&](int o, int t) {
+                return at(tb.ac_a[t], tb.bd_b[o], tb.ac_c[t], tb.bd_d[o]) * S[0][t];
+            }, [&](int o, double v) { unsafeAtomicAdd(&Bs[(size_t)(pb.f1 + tb.bd_b[o]) * nb + pk.f1 + tb.bd_d[o]], v); });
+        }
         __syncthreads();
     }
 }

@@ -36,10 +36,14 @@ struct JKArgs {
     const PrimDev *__restrict__ prims;
     const long long *__restrict__ mol_off;
     const int *__restrict__ mol_nb;
-    const double *__restrict__ P;  // density scaled by component norms
-    const double *__restrict__ Q;  // Schwarz per function pair (normalised functions)
-    double *A;                     // J accumulator, one orientation
-    double *B;                     // K accumulator, one orientation
+    const double *__restrict__ P;    // density of J (RHF: also of K), scaled by component norms
+    const float *__restrict__ P32;   // the same density in FP32 (read by the FP32 kernels)
+    const double *__restrict__ PK;   // one-pass UHF: exchange density of B (Pa), scaled
+    const double *__restrict__ PK2;  // one-pass UHF: exchange density of B2 (Pb), scaled
+    const double *__restrict__ Q;    // Schwarz per function pair (normalised functions)
+    double *A;                       // J accumulator, one orientation
+    double *B;                       // K accumulator, one orientation
+    double *B2;                      // second K accumulator (one-pass UHF)
     double tol;
     int nwaves;
     int remap;  // 1: each XCD takes a contiguous range of waves (grid a multiple of 8)
@@ -110,8 +114,10 @@ __device__ __forceinline__ void os_values(const PairDev &pb, const PairDev &pk, 
 }
 
 // J/K kernel: one wave = one bra pair and up to 64 ket pairs.  The integrals
-// are computed in R; digestion into J and K is FP64.
-template <typename R, int LA, int LB, int LC, int LD, int ALG, int C0, int C1>
+// are computed in R; the J and K accumulators are FP64.
+// NK = 2: one-pass UHF digestion, J of P (= Pa + Pb) into A and the exchange of
+// PK (= Pa) into B and of PK2 (= Pb) into B2 from the same integrals.
+template <typename R, int LA, int LB, int LC, int LD, int ALG, int C0, int C1, int NK = 1>
 __global__ __launch_bounds__(256) void k_jk(JKArgs a) {
     using K = Cls<LA, LB, LC, LD>;
     constexpr int NA = K::NA, NB = K::NB, ND = K::ND, NAB = K::NAB, NE = K::NE, NCH = C1 - C0;
@@ -120,10 +126,19 @@ __global__ __launch_bounds__(256) void k_jk(JKArgs a) {
     const int mol = a.wave_mol[wave];
     const long long moff = a.mol_off[mol];
     const int nb = a.mol_nb[mol];
-    const double *__restrict__ P = a.P + moff;
+    // per-quartet digestion sums in D: FP64, or FP32 for the FP32 kernels (the
+    // quartets below the FP32 cutoff), which then read an FP32 copy of the
+    // density; the J and K accumulators themselves are FP64 (atomics)
+    using D = std::conditional_t<std::is_same_v<R, float>, float, double>;
+    const D *__restrict__ P = std::is_same_v<D, float> ? (const D *)(const void *)(a.P32 + moff)
+                                                       : (const D *)(const void *)(a.P + moff);
+    static_assert(NK == 1 || std::is_same_v<D, double>, "two-density digestion is FP64");
+    const D *__restrict__ PK = NK == 2 ? (const D *)(const void *)(a.PK + moff) : P;   // exchange density of B
+    const D *__restrict__ P2 = NK == 2 ? (const D *)(const void *)(a.PK2 + moff) : P;  // exchange density of B2
     const double *__restrict__ Qm = a.Q + moff;
     double *Am = a.A + moff;
     double *Bm = a.B + moff;
+    double *B2m = NK == 2 ? a.B2 + moff : Bm;
     const int lane = threadIdx.x & 63;
     const int bra = a.wave_bra[wave];
     const int cnt = a.wave_count[wave];
@@ -133,19 +148,28 @@ __global__ __launch_bounds__(256) void k_jk(JKArgs a) {
     const PairDev pk = a.pairs[ket];
     const int nq = active ? pk.nprim : 0;
 
-    double deg = 1.0;
-    if (pb.diag) deg *= 0.5;
-    if (pk.diag) deg *= 0.5;
-    if (bra == ket) deg *= 0.5;
-    if (!active) deg = 0.0;
+    D deg = D(1);
+    if (pb.diag) deg *= D(0.5);
+    if (pk.diag) deg *= D(0.5);
+    if (bra == ket) deg *= D(0.5);
+    if (!active) deg = D(0);
     // every function quartet passes the Schwarz test when the smallest
     // function-pair bounds do; otherwise each one is tested
     const bool all_pass = pb.qmin * pk.qmin >= a.tol;
 
-    double jab[NAB], kac[NA], kbc[NB], kad[NA][ND], kbd[NB][ND];
-    sfor<0, NAB>([&](auto i_) { jab[PK_CI(i_)] = 0.0; });
-    sfor<0, NA>([&](auto i_) { kac[PK_CI(i_)] = 0.0; sfor<0, ND>([&](auto d_) { kad[PK_CI(i_)][PK_CI(d_)] = 0.0; }); });
-    sfor<0, NB>([&](auto i_) { kbc[PK_CI(i_)] = 0.0; sfor<0, ND>([&](auto d_) { kbd[PK_CI(i_)][PK_CI(d_)] = 0.0; }); });
+    D jab[NAB], kac[NK][NA], kbc[NK][NB], kad[NK][NA][ND], kbd[NK][NB][ND];
+    sfor<0, NAB>([&](auto i_) { jab[PK_CI(i_)] = D(0); });
+    sfor<0, NK>([&](auto s_) {
+        constexpr int sp = PK_CI(s_);
+        sfor<0, NA>([&](auto i_) {
+            kac[sp][PK_CI(i_)] = D(0);
+            sfor<0, ND>([&](auto d_) { kad[sp][PK_CI(i_)][PK_CI(d_)] = D(0); });
+        });
+        sfor<0, NB>([&](auto i_) {
+            kbc[sp][PK_CI(i_)] = D(0);
+            sfor<0, ND>([&](auto d_) { kbd[sp][PK_CI(i_)][PK_CI(d_)] = D(0); });
+        });
+    });
 
     // The work lists order each bra's kets by ordering group (a bucket of
     // primitive counts), then first shell, then second shell, so lanes with
@@ -162,9 +186,9 @@ __global__ __launch_bounds__(256) void k_jk(JKArgs a) {
     });
     const int down = __shfl_down(ckey, 1, 64);
     const bool seg_tail = lane == 63 || down != ckey;
-    auto seg_sum = [&](double v) {
+    auto seg_sum = [&](D v) {
         sfor<0, 6>([&](auto k_) {
-            const double o = __shfl_up(v, 1 << PK_CI(k_), 64);
+            const D o = __shfl_up(v, 1 << PK_CI(k_), 64);
             if (seg_same[PK_CI(k_)]) v += o;
         });
         return v;
@@ -175,41 +199,54 @@ __global__ __launch_bounds__(256) void k_jk(JKArgs a) {
         constexpr int jj = PK_CI(jj_);
         constexpr int ic = (C0 + jj) / ND, id = (C0 + jj) % ND;
         const int cf = pk.f0 + ic, df = pk.f1 + id;
-        const double Pcd = P[(size_t)cf * nb + df];
+        const D Pcd = P[(size_t)cf * nb + df];
         const double Qcd = Qm[(size_t)cf * nb + df];
-        double jcd = 0.0;
+        D jcd = D(0);
         sfor<0, NA>([&](auto ia_) {
             constexpr int ia = PK_CI(ia_);
             const int af = pb.f0 + ia;
-            const double Pac = P[(size_t)af * nb + cf], Pad = P[(size_t)af * nb + df];
+            const D *KD[2] = {PK, P2};
+            D Pac[NK], Pad[NK];
+            sfor<0, NK>([&](auto s_) {
+                constexpr int sp = PK_CI(s_);
+                Pac[sp] = KD[sp][(size_t)af * nb + cf];
+                Pad[sp] = KD[sp][(size_t)af * nb + df];
+            });
             sfor<0, NB>([&](auto ib_) {
                 constexpr int ib = PK_CI(ib_);
                 const int bf = pb.f1 + ib;
                 const size_t abi = (size_t)af * nb + bf;
-                const double m = (all_pass || Qm[abi] * Qcd >= a.tol) ? deg : 0.0;
-                const double dv = m * (double)v[ia * NB + ib];
+                const D m = (all_pass || Qm[abi] * Qcd >= a.tol) ? deg : D(0);
+                const D dv = m * (D)v[ia * NB + ib];
                 jcd = fma(dv, P[abi], jcd);
                 jab[ia * NB + ib] = fma(dv, Pcd, jab[ia * NB + ib]);
-                kac[ia] = fma(dv, P[(size_t)bf * nb + df], kac[ia]);
-                kad[ia][id] = fma(dv, P[(size_t)bf * nb + cf], kad[ia][id]);
-                kbc[ib] = fma(dv, Pad, kbc[ib]);
-                kbd[ib][id] = fma(dv, Pac, kbd[ib][id]);
+                sfor<0, NK>([&](auto s_) {
+                    constexpr int sp = PK_CI(s_);
+                    kac[sp][ia] = fma(dv, KD[sp][(size_t)bf * nb + df], kac[sp][ia]);
+                    kad[sp][ia][id] = fma(dv, KD[sp][(size_t)bf * nb + cf], kad[sp][ia][id]);
+                    kbc[sp][ib] = fma(dv, Pad[sp], kbc[sp][ib]);
+                    kbd[sp][ib][id] = fma(dv, Pac[sp], kbd[sp][ib][id]);
+                });
             });
         });
-        if (active) unsafeAtomicAdd(&Am[(size_t)cf * nb + df], 2.0 * jcd);
+        if (active) unsafeAtomicAdd(&Am[(size_t)cf * nb + df], 2.0 * (double)jcd);
         // last ket component of this c in the chunk: flush the (a|c), (b|c)
         // sums, added first over the lanes that share the ket's first shell
         if constexpr (id == ND - 1 || jj == NCH - 1) {
-            sfor<0, NA>([&](auto ia_) {
-                const double v = seg_sum(kac[PK_CI(ia_)]);
-                if (active && seg_tail) unsafeAtomicAdd(&Bm[(size_t)(pb.f0 + PK_CI(ia_)) * nb + cf], v);
+            sfor<0, NK>([&](auto s_) {
+                constexpr int sp = PK_CI(s_);
+                double *Bs = sp == 0 ? Bm : B2m;
+                sfor<0, NA>([&](auto ia_) {
+                    const D v = seg_sum(kac[sp][PK_CI(ia_)]);
+                    if (active && seg_tail) unsafeAtomicAdd(&Bs[(size_t)(pb.f0 + PK_CI(ia_)) * nb + cf], (double)v);
+                });
+                sfor<0, NB>([&](auto ib_) {
+                    const D v = seg_sum(kbc[sp][PK_CI(ib_)]);
+                    if (active && seg_tail) unsafeAtomicAdd(&Bs[(size_t)(pb.f1 + PK_CI(ib_)) * nb + cf], (double)v);
+                });
+                sfor<0, NA>([&](auto i_) { kac[sp][PK_CI(i_)] = D(0); });
+                sfor<0, NB>([&](auto i_) { kbc[sp][PK_CI(i_)] = D(0); });
             });
-            sfor<0, NB>([&](auto ib_) {
-                const double v = seg_sum(kbc[PK_CI(ib_)]);
-                if (active && seg_tail) unsafeAtomicAdd(&Bm[(size_t)(pb.f1 + PK_CI(ib_)) * nb + cf], v);
-            });
-            sfor<0, NA>([&](auto i_) { kac[PK_CI(i_)] = 0.0; });
-            sfor<0, NB>([&](auto i_) { kbc[PK_CI(i_)] = 0.0; });
         }
     };
 
@@ -242,18 +279,22 @@ __global__ __launch_bounds__(256) void k_jk(JKArgs a) {
             }();
             if constexpr (present) {
                 const int df = pk.f1 + id;
-                sfor<0, NA>([&](auto ia_) {
-                    unsafeAtomicAdd(&Bm[(size_t)(pb.f0 + PK_CI(ia_)) * nb + df], kad[PK_CI(ia_)][id]);
-                });
-                sfor<0, NB>([&](auto ib_) {
-                    unsafeAtomicAdd(&Bm[(size_t)(pb.f1 + PK_CI(ib_)) * nb + df], kbd[PK_CI(ib_)][id]);
+                sfor<0, NK>([&](auto s_) {
+                    constexpr int sp = PK_CI(s_);
+                    double *Bs = sp == 0 ? Bm : B2m;
+                    sfor<0, NA>([&](auto ia_) {
+                        unsafeAtomicAdd(&Bs[(size_t)(pb.f0 + PK_CI(ia_)) * nb + df], (double)kad[sp][PK_CI(ia_)][id]);
+                    });
+                    sfor<0, NB>([&](auto ib_) {
+                        unsafeAtomicAdd(&Bs[(size_t)(pb.f1 + PK_CI(ib_)) * nb + df], (double)kbd[sp][PK_CI(ib_)][id]);
+                    });
                 });
             }
         });
     }
     sfor<0, NAB>([&](auto i_) {
         constexpr int i = PK_CI(i_);
-        const double s = wave_sum(jab[i]);
+        const double s = wave_sum((double)jab[i]);
         if (lane == 0) unsafeAtomicAdd(&Am[(size_t)(pb.f0 + i / NB) * nb + pb.f1 + i % NB], 2.0 * s);
     });
 }

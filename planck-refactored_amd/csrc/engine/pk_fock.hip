@@ -33,7 +33,8 @@
     void pk_launch_jk_rys_##a##b##c##d(const pk::JKArgs &, int, int, hipStream_t, int &);           \
     void pk_launch_jk_os_##a##b##c##d(const pk::JKArgs &, int, int, hipStream_t, int &);            \
     void pk_launch_jk_rys32_##a##b##c##d(const pk::JKArgs &, int, int, hipStream_t, int &);         \
-    void pk_launch_jk_os32_##a##b##c##d(const pk::JKArgs &, int, int, hipStream_t, int &);
+    void pk_launch_jk_os32_##a##b##c##d(const pk::JKArgs &, int, int, hipStream_t, int &);         \
+    void pk_launch_jk_os_uhf_##a##b##c##d(const pk::JKArgs &, int, int, hipStream_t, int &);
 PK_FOR_EACH_CLASS(PK_DECL)
 #undef PK_DECL
 
@@ -66,12 +67,14 @@ struct ClassFns {
     void (*jk_rys)(const pk::JKArgs &, int, int, hipStream_t, int &);
     void (*jk_os32)(const pk::JKArgs &, int, int, hipStream_t, int &);  // FP32 integrals
     void (*jk_rys32)(const pk::JKArgs &, int, int, hipStream_t, int &);
+    void (*jk_os_uhf)(const pk::JKArgs &, int, int, hipStream_t, int &);  // FP64, two exchange densities
 };
 
 #define PK_ENTRY(a, b, c, d, shard)                                                             \
     ClassFns{a, b, c, d, pk::os_available(a, b, c, d), pk::rys_tmpl_available(a, b, c, d),      \
              &pk_launch_jk_os_##a##b##c##d, &pk_launch_jk_rys_##a##b##c##d,                     \
-             &pk_launch_jk_os32_##a##b##c##d, &pk_launch_jk_rys32_##a##b##c##d},
+             &pk_launch_jk_os32_##a##b##c##d, &pk_launch_jk_rys32_##a##b##c##d,                 \
+             &pk_launch_jk_os_uhf_##a##b##c##d},
 // Quartet classes (bra class >= ket class), indexed cb (cb + 1) / 2 + ck with
 // pair classes (0,0) (1,0) (1,1) (2,0) (2,1) (2,2) (3,0) (3,1) (3,2) (3,3).
 const ClassFns kClasses[kQuartClasses] = {PK_FOR_EACH_CLASS(PK_ENTRY)};
@@ -128,10 +131,16 @@ __global__ void k_unpermute(const double *__restrict__ in, double *__restrict__ 
     out[o + i] = in[o + (size_t)ps[r] * nb + ps[c]];
 }
 
-// x += y
-__global__ void k_accumulate(double *__restrict__ x, const double *__restrict__ y, size_t n) {
+// y = (float) x
+__global__ void k_to_f32(const double *__restrict__ x, float *__restrict__ y, size_t n) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) x[i] += y[i];
+    if (i < n) y[i] = (float)x[i];
+}
+
+// z = x + y
+__global__ void k_sum2(const double *__restrict__ x, const double *__restrict__ y, double *__restrict__ z, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) z[i] = x[i] + y[i];
 }
 
 __global__ void k_fill(double *__restrict__ x, size_t n, double v) {
@@ -161,7 +170,7 @@ __global__ void k_finalize(const double *__restrict__ A, const double *__restric
 }
 
 struct TuneEntry {
-    int cls, prec, block, remap;
+    int cls, prec, block, remap, alt;
 };
 
 struct ClassWork {
@@ -169,6 +178,8 @@ struct ClassWork {
     int prec = PK_PREC_FP64;  // PK_PREC_MIXED: the FP32 list of the class
     int block = 256;          // lanes per workgroup (autotuned)
     int remap = 0;            // XCD-contiguous wave ranges (autotuned)
+    int alt = 0;              // alternative kernel (autotuned): FP64 integrals for an FP32 list,
+                              // the cooperative kernel for a per-lane Rys class
     int nwaves = 0;
     int64_t nquart = 0;
     int64_t nprimq = 0;
@@ -230,7 +241,9 @@ struct pk_ctx {
     double *d_Q = nullptr, *d_fnorm = nullptr, *d_P = nullptr, *d_Pn = nullptr, *d_A = nullptr, *d_B = nullptr;
     double *d_J = nullptr, *d_K = nullptr;
     double *d_J2 = nullptr;  // UHF: J of the second density
-    double *d_palpha = nullptr, *d_Pk1n = nullptr, *d_Pk2n = nullptr, *d_gsh = nullptr;  // gradient
+    double *d_palpha = nullptr, *d_Pk1n = nullptr, *d_Pk2n = nullptr, *d_gsh = nullptr;  // gradient, UHF
+    double *d_B2 = nullptr;  // one-pass UHF: second exchange accumulator
+    float *d_P32 = nullptr;  // mixed precision: FP32 copy of the scaled density
     int *d_pair_sh = nullptr;
     std::vector<ClassWork> work;
     bool work_ok = false;
@@ -249,6 +262,16 @@ namespace {
 
 // OS kernels where they exist unless the Rys engine is forced
 bool use_os(const pk_ctx *c, const ClassFns &f) { return f.has_os && c->engine != PK_ENGINE_RYS; }
+
+// Kernels that run a class list (pk_class_stats.kind): 0 / 3 per-lane OS in
+// FP64 / FP32, 1 / 4 per-lane Rys in FP64 / FP32, 2 cooperative (FP64).
+int list_kind(const pk_ctx *c, const ClassWork &w) {
+    const ClassFns &f = kClasses[w.cls];
+    const bool fp32 = w.prec == PK_PREC_MIXED && !w.alt;
+    if (use_os(c, f)) return fp32 ? 3 : 0;
+    if (f.has_tmpl && !(w.alt && w.prec == PK_PREC_FP64)) return fp32 ? 4 : 1;
+    return 2;
+}
 
 void drop_graph(pk_ctx *c) {
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
@@ -277,11 +300,17 @@ void free_work(pk_ctx *c) {
 int launch_generic(const pk::GenArgs &a, hipStream_t s) {
     if (a.nitems <= 0) return PK_OK;
     static const size_t max_bytes = pk::generic_lds_bytes(kMaxL, kMaxL, kMaxL, kMaxL);
-    static const hipError_t attr =
-        hipFuncSetAttribute((const void *)pk::k_generic, hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_bytes);
-    PK_HIP(attr);
+    static const hipError_t attr1 =
+        hipFuncSetAttribute((const void *)pk::k_generic<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_bytes);
+    static const hipError_t attr2 =
+        hipFuncSetAttribute((const void *)pk::k_generic<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_bytes);
+    PK_HIP(attr1);
+    PK_HIP(attr2);
     const size_t bytes = pk::generic_lds_bytes(a.la, a.lb, a.lc, a.ld);
-    hipLaunchKernelGGL(pk::k_generic, dim3(a.nitems), dim3(64), bytes, s, a);
+    if (a.B2)
+        hipLaunchKernelGGL(pk::k_generic<2>, dim3(a.nitems), dim3(64), bytes, s, a);
+    else
+        hipLaunchKernelGGL(pk::k_generic<1>, dim3(a.nitems), dim3(64), bytes, s, a);
     PK_HIP(hipGetLastError());
     return PK_OK;
 }
@@ -509,6 +538,7 @@ int build_work(pk_ctx *c) {
             if (t.cls == w.cls && t.prec == w.prec) {
                 w.block = t.block;
                 w.remap = t.remap;
+                w.alt = t.alt;
             }
     PK_HIP(hipStreamSynchronize(c->stream));
     c->work_ok = true;
@@ -524,6 +554,7 @@ int ensure_buffers(pk_ctx *c) {
     if (!c->d_J) PK_HIP(hipMalloc((void **)&c->d_J, n2));
     if (!c->d_K) PK_HIP(hipMalloc((void **)&c->d_K, n2));
     if (!c->d_G) PK_HIP(hipMalloc((void **)&c->d_G, n2));
+    if (!c->d_P32 && c->precision == PK_PREC_MIXED) PK_HIP(hipMalloc((void **)&c->d_P32, c->n2 * sizeof(float)));
     return PK_OK;
 }
 
@@ -532,28 +563,48 @@ int ensure_buffers(pk_ctx *c) {
 // (never while the stream is being captured).
 // Per-build scratch: the scaled density and the J, K accumulators (packed,
 // c->n2 doubles each).
+// One-pass UHF adds the scaled exchange densities PKa, PKb and the second K
+// accumulator B2 (all null for RHF; Pn then holds Pa + Pb).
+// P32 is an FP32 copy of Pn for the FP32 kernels of the mixed precision.
 struct BuildBufs {
     double *Pn, *A, *B;
+    float *P32 = nullptr;
+    double *PKa = nullptr, *PKb = nullptr, *B2 = nullptr;
 };
 
-BuildBufs ctx_bufs(pk_ctx *c) { return BuildBufs{c->d_Pn, c->d_A, c->d_B}; }
+BuildBufs ctx_bufs(pk_ctx *c) { return BuildBufs{c->d_Pn, c->d_A, c->d_B, c->d_P32}; }
 
-int launch_build(pk_ctx *c, const double *dP, double *dJ, double *dK, double *dG, const BuildBufs &sb,
-                 bool events) {
+// Enqueue the class kernels of one build (the density already scaled into
+// sb.Pn, the accumulators cleared); nl counts the launches.  With sb.B2 set
+// (one-pass UHF) every list runs FP64 two-density digestion: the per-lane OS
+// kernels where the class has them, else the cooperative kernel.
+int launch_classes(pk_ctx *c, const BuildBufs &sb, bool events, int &nl) {
     int rc;
     hipStream_t s = c->stream;
-    const MolTab t{c->d_mol_off, c->d_mol_nb, c->d_mol_foff, c->d_fpos};
-    const dim3 g2((unsigned)(((size_t)c->nbmax * c->nbmax + 255) / 256), (unsigned)c->nmol);
-    const int gf = (int)((c->n2 + 255) / 256);
-    hipLaunchKernelGGL(k_scale_density, g2, dim3(256), 0, s, dP, c->d_fnorm, sb.Pn, t);
-    hipLaunchKernelGGL(k_fill, dim3(gf), dim3(256), 0, s, sb.A, c->n2, 0.0);
-    hipLaunchKernelGGL(k_fill, dim3(gf), dim3(256), 0, s, sb.B, c->n2, 0.0);
-    int nl = 3;
     for (ClassWork &w : c->work) {
         const int *wv = w.d_wave;
-        pk::JKArgs a{wv, wv + w.nwaves, wv + 2 * w.nwaves, wv + 3 * w.nwaves, w.d_kets, c->d_pairs,
-                     c->d_prims, c->d_mol_off, c->d_mol_nb, sb.Pn, c->d_Q, sb.A, sb.B,
-                     c->tol, w.nwaves, w.remap};
+        pk::JKArgs a{};
+        a.wave_bra = wv;
+        a.wave_start = wv + w.nwaves;
+        a.wave_count = wv + 2 * w.nwaves;
+        a.wave_mol = wv + 3 * w.nwaves;
+        a.kets = w.d_kets;
+        a.pairs = c->d_pairs;
+        a.prims = c->d_prims;
+        a.mol_off = c->d_mol_off;
+        a.mol_nb = c->d_mol_nb;
+        a.P = sb.Pn;
+        a.P32 = sb.P32;
+        a.PK = sb.PKa;
+        a.PK2 = sb.PKb;
+        a.Q = c->d_Q;
+        a.A = sb.A;
+        a.B = sb.B;
+        a.B2 = sb.B2;
+        a.tol = c->tol;
+        a.nwaves = w.nwaves;
+        a.remap = w.remap;
+        const bool uhf = sb.B2 != nullptr;
         if (events) {
             if (!w.e0) PK_HIP(hipEventCreate(&w.e0));
             if (!w.e1) PK_HIP(hipEventCreate(&w.e1));
@@ -561,15 +612,30 @@ int launch_build(pk_ctx *c, const double *dP, double *dJ, double *dK, double *dG
         }
         const int before = nl;
         const ClassFns &f = kClasses[w.cls];
-        const bool fp32 = w.prec == PK_PREC_MIXED;
+        const bool fp32 = w.prec == PK_PREC_MIXED && !w.alt;
         const int wpb = w.block / 64;
         int grid = (w.nwaves + wpb - 1) / wpb;
         if (w.remap) grid = (grid + 7) / 8 * 8;
-        if (use_os(c, f)) {
+        if (uhf && use_os(c, f)) {
+            f.jk_os_uhf(a, grid, w.block, s, nl);
+        } else if (uhf && f.has_tmpl) {
+            // per-lane Rys kernels once per spin density: J of each into A
+            // (summing to J of Pa + Pb), exchange into B and B2
+            pk::JKArgs x = a;
+            x.P = sb.PKa;
+            f.jk_rys(x, grid, w.block, s, nl);
+            x.P = sb.PKb;
+            x.B = sb.B2;
+            f.jk_rys(x, grid, w.block, s, nl);
+        } else if (uhf) {
+            // handled by the cooperative kernel below
+        } else if (use_os(c, f)) {
             (fp32 ? f.jk_os32 : f.jk_os)(a, grid, w.block, s, nl);
-        } else if (f.has_tmpl) {
+        } else if (f.has_tmpl && !(w.alt && w.prec == PK_PREC_FP64)) {
             (fp32 ? f.jk_rys32 : f.jk_rys)(a, grid, w.block, s, nl);
-        } else {
+        }
+        if ((uhf && !use_os(c, f) && !f.has_tmpl) ||
+            (!uhf && !use_os(c, f) && !(f.has_tmpl && !(w.alt && w.prec == PK_PREC_FP64)))) {
             pk::GenArgs g{};
             g.wave_bra = a.wave_bra;
             g.wave_start = a.wave_start;
@@ -581,9 +647,12 @@ int launch_build(pk_ctx *c, const double *dP, double *dJ, double *dK, double *dG
             g.pairs = a.pairs;
             g.prims = a.prims;
             g.P = a.P;
+            g.PK = uhf ? a.PK : a.P;
+            g.PK2 = a.PK2;
             g.Q = a.Q;
             g.A = a.A;
             g.B = a.B;
+            g.B2 = a.B2;
             g.tol = a.tol;
             g.nitems = w.nwaves;
             g.la = f.la;
@@ -596,8 +665,54 @@ int launch_build(pk_ctx *c, const double *dP, double *dJ, double *dK, double *dG
         w.kernels = nl - before;
         if (events) PK_HIP(hipEventRecord(w.e1, s));
     }
+    return PK_OK;
+}
+
+int launch_build(pk_ctx *c, const double *dP, double *dJ, double *dK, double *dG, const BuildBufs &sb,
+                 bool events) {
+    int rc;
+    hipStream_t s = c->stream;
+    const MolTab t{c->d_mol_off, c->d_mol_nb, c->d_mol_foff, c->d_fpos};
+    const dim3 g2((unsigned)(((size_t)c->nbmax * c->nbmax + 255) / 256), (unsigned)c->nmol);
+    const int gf = (int)((c->n2 + 255) / 256);
+    hipLaunchKernelGGL(k_scale_density, g2, dim3(256), 0, s, dP, c->d_fnorm, sb.Pn, t);
+    hipLaunchKernelGGL(k_fill, dim3(gf), dim3(256), 0, s, sb.A, c->n2, 0.0);
+    hipLaunchKernelGGL(k_fill, dim3(gf), dim3(256), 0, s, sb.B, c->n2, 0.0);
+    int nl = 3;
+    if (c->precision == PK_PREC_MIXED) {
+        if (!sb.P32) return fail(PK_ERR_STATE, "mixed precision build without an FP32 density buffer");
+        hipLaunchKernelGGL(k_to_f32, dim3(gf), dim3(256), 0, s, sb.Pn, sb.P32, c->n2);
+        ++nl;
+    }
+    if ((rc = launch_classes(c, sb, events, nl)) != PK_OK) return rc;
     hipLaunchKernelGGL(k_finalize, g2, dim3(256), 0, s, sb.A, sb.B, c->d_fnorm, dJ, dK, dG, t);
     ++nl;
+    PK_HIP(hipGetLastError());
+    c->st.kernel_launches = nl;
+    return PK_OK;
+}
+
+// One-pass UHF build: Pn = n n^T o (Pa + Pb) for J, PKa, PKb the scaled spin
+// densities for the two exchange matrices; each integral computed once.
+int launch_build_uhf(pk_ctx *c, const double *dPa, const double *dPb, double *dJt, double *dKa, double *dKb,
+                     const BuildBufs &sb, bool events) {
+    int rc;
+    hipStream_t s = c->stream;
+    const MolTab t{c->d_mol_off, c->d_mol_nb, c->d_mol_foff, c->d_fpos};
+    const dim3 g2((unsigned)(((size_t)c->nbmax * c->nbmax + 255) / 256), (unsigned)c->nmol);
+    const int gf = (int)((c->n2 + 255) / 256);
+    hipLaunchKernelGGL(k_scale_density, g2, dim3(256), 0, s, dPa, c->d_fnorm, sb.PKa, t);
+    hipLaunchKernelGGL(k_scale_density, g2, dim3(256), 0, s, dPb, c->d_fnorm, sb.PKb, t);
+    hipLaunchKernelGGL(k_sum2, dim3(gf), dim3(256), 0, s, sb.PKa, sb.PKb, sb.Pn, c->n2);
+    hipLaunchKernelGGL(k_fill, dim3(gf), dim3(256), 0, s, sb.A, c->n2, 0.0);
+    hipLaunchKernelGGL(k_fill, dim3(gf), dim3(256), 0, s, sb.B, c->n2, 0.0);
+    hipLaunchKernelGGL(k_fill, dim3(gf), dim3(256), 0, s, sb.B2, c->n2, 0.0);
+    int nl = 6;
+    if ((rc = launch_classes(c, sb, events, nl)) != PK_OK) return rc;
+    hipLaunchKernelGGL(k_finalize, g2, dim3(256), 0, s, sb.A, sb.B, c->d_fnorm, dJt, dKa, (double *)nullptr, t);
+    hipLaunchKernelGGL(k_finalize, g2, dim3(256), 0, s, sb.A, sb.B2, c->d_fnorm, (double *)nullptr, dKb,
+                       (double *)nullptr, t);
+    nl += 2;
     PK_HIP(hipGetLastError());
     c->st.kernel_launches = nl;
     return PK_OK;
@@ -676,7 +791,8 @@ int pk_build_with_scratch(pk_ctx *c, const double *P, double *J, double *K, doub
     if (!c->work_ok)
         return fail(PK_ERR_STATE, who + ": the work lists of the current settings are not built "
                                         "(call pk_fock_jk once after pk_ctx_create / pk_ctx_set)");
-    const BuildBufs sb{scratch, scratch + each, scratch + 2 * each};
+    BuildBufs sb{scratch, scratch + each, scratch + 2 * each};
+    sb.P32 = reinterpret_cast<float *>(scratch + 3 * each);
     return launch_build(c, P, J, K, nullptr, sb, false);
 }
 
@@ -932,7 +1048,10 @@ int pk_ctx_autotune(pk_ctx *c, int reps) {
     hipLaunchKernelGGL(k_fill, dim3((unsigned)((c->n2 + 255) / 256)), dim3(256), 0, s, c->d_P, c->n2, 0.01);
     const bool prof = c->profile;
     c->profile = true;
-    const int cand[6][2] = {{64, 0}, {128, 0}, {256, 0}, {64, 1}, {128, 1}, {256, 1}};
+    // launch shapes (lanes per workgroup, XCD remap) of the class kernels, and
+    // the alternative kernel of the list (see ClassWork::alt) in its own shape
+    constexpr int kCand = 7;
+    const int cand[kCand][3] = {{64, 0, 0}, {128, 0, 0}, {256, 0, 0}, {64, 1, 0}, {128, 1, 0}, {256, 1, 0}, {256, 0, 1}};
     const size_t nw = c->work.size();
     std::vector<double> best(nw, 1e300);
     std::vector<int> pick(nw, 2);
@@ -940,6 +1059,7 @@ int pk_ctx_autotune(pk_ctx *c, int reps) {
         for (ClassWork &w : c->work) {
             w.block = cand[k][0];
             w.remap = cand[k][1];
+            w.alt = cand[k][2];
         }
         std::vector<double> t(nw, 0.0);
         for (int r = 0; r < nrep; ++r) {
@@ -961,7 +1081,7 @@ int pk_ctx_autotune(pk_ctx *c, int reps) {
         return PK_OK;
     };
     rc = run(2, 1, false);
-    for (int k = 0; k < 6 && rc == PK_OK; ++k) rc = run(k, reps, true);
+    for (int k = 0; k < kCand && rc == PK_OK; ++k) rc = run(k, reps, true);
     c->profile = prof;
     if (rc != PK_OK) return rc;
     c->tuned.clear();
@@ -969,9 +1089,10 @@ int pk_ctx_autotune(pk_ctx *c, int reps) {
         ClassWork &w = c->work[i];
         w.block = cand[pick[i]][0];
         w.remap = cand[pick[i]][1];
+        w.alt = cand[pick[i]][2];
         w.ms_total = 0.0;
         w.builds = 0;
-        c->tuned.push_back(TuneEntry{w.cls, w.prec, w.block, w.remap});
+        c->tuned.push_back(TuneEntry{w.cls, w.prec, w.block, w.remap, w.alt});
     }
     drop_graph(c);
     return PK_OK;
@@ -990,7 +1111,7 @@ int pk_ctx_destroy(pk_ctx *c) {
     free_work(c);
     for (void *p : {(void *)c->d_pairs, (void *)c->d_prims, (void *)c->d_Q, (void *)c->d_fnorm, (void *)c->d_P,
                     (void *)c->d_Pn, (void *)c->d_A, (void *)c->d_B, (void *)c->d_J, (void *)c->d_K, (void *)c->d_G, (void *)c->d_J2,
-                    (void *)c->d_palpha, (void *)c->d_Pk1n, (void *)c->d_Pk2n, (void *)c->d_gsh, (void *)c->d_pair_sh,
+                    (void *)c->d_palpha, (void *)c->d_B2, (void *)c->d_P32, (void *)c->d_Pk1n, (void *)c->d_Pk2n, (void *)c->d_gsh, (void *)c->d_pair_sh,
                     (void *)c->d_mol_off, (void *)c->d_mol_nb, (void *)c->d_mol_foff, (void *)c->d_fpos})
         if (p) (void)hipFree(p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -1068,6 +1189,8 @@ int pk_ctx_class_stats(const pk_ctx *c, int i, pk_class_stats *o) {
     o->precision = w.prec;
     o->block = w.block;
     o->remap = w.remap;
+    o->alt = w.alt;
+    o->kind = list_kind(c, w);
     o->waves = w.nwaves;
     return PK_OK;
 }
@@ -1111,34 +1234,35 @@ int pk_fock_jk_uhf(pk_ctx *c, const double *Pa, const double *Pb, double *Jt, do
     if (!c || !Pa || !Pb) return fail(PK_ERR_ARG, "pk_fock_jk_uhf: null argument");
     if (memkind != PK_MEM_HOST && memkind != PK_MEM_DEVICE) return fail(PK_ERR_ARG, "pk_fock_jk_uhf: bad memkind");
     int rc;
+    if (!c->work_ok && (rc = build_work(c)) != PK_OK) return rc;
     if ((rc = ensure_buffers(c)) != PK_OK) return rc;
     const size_t n2 = c->n2 * sizeof(double);
     if (!c->d_J2) PK_HIP(hipMalloc((void **)&c->d_J2, n2));
+    if (!c->d_Pk1n) PK_HIP(hipMalloc((void **)&c->d_Pk1n, n2));
+    if (!c->d_Pk2n) PK_HIP(hipMalloc((void **)&c->d_Pk2n, n2));
+    if (!c->d_B2) PK_HIP(hipMalloc((void **)&c->d_B2, n2));
     hipStream_t s = c->stream;
-    const unsigned g = (unsigned)((c->n2 + 255) / 256);
-    // two builds on the device: (J[Pa], K[Pa]) and (J[Pb], K[Pb]); Jt = J[Pa] + J[Pb]
+    // one pass over the integrals: J of Pa + Pb, the exchange of each spin
+    // density from the same integrals (two exchange accumulators)
+    BuildBufs sb = ctx_bufs(c);
+    sb.PKa = c->d_Pk1n;
+    sb.PKb = c->d_Pk2n;
+    sb.B2 = c->d_B2;
+    PK_HIP(hipEventRecord(c->ev0, s));
     if (memkind == PK_MEM_DEVICE) {
-        if ((rc = build_device(c, Pa, Jt ? Jt : nullptr, Ka, nullptr)) != PK_OK) return rc;
-        if ((rc = build_device(c, Pb, Jt ? c->d_J2 : nullptr, Kb, nullptr)) != PK_OK) return rc;
-        if (Jt) {
-            hipLaunchKernelGGL(k_accumulate, dim3(g), dim3(256), 0, s, Jt, c->d_J2, c->n2);
-            PK_HIP(hipGetLastError());
-        }
+        if ((rc = launch_build_uhf(c, Pa, Pb, Jt, Ka, Kb, sb, c->profile)) != PK_OK) return rc;
+        PK_HIP(hipEventRecord(c->ev1, s));
         PK_HIP(hipStreamSynchronize(s));
         return finish_timing(c);
     }
-    // host memory: the context's buffers (a captured build writes d_J / d_K)
+    // host memory: staged through the context's buffers
     PK_HIP(hipMemcpyAsync(c->d_P, Pa, n2, hipMemcpyHostToDevice, s));
-    if ((rc = build_device(c, c->d_P, c->d_J, c->d_K, nullptr)) != PK_OK) return rc;
-    if (Ka) PK_HIP(hipMemcpyAsync(Ka, c->d_K, n2, hipMemcpyDeviceToHost, s));
-    PK_HIP(hipMemcpyAsync(c->d_J2, c->d_J, n2, hipMemcpyDeviceToDevice, s));
-    PK_HIP(hipStreamSynchronize(s));
-    PK_HIP(hipMemcpyAsync(c->d_P, Pb, n2, hipMemcpyHostToDevice, s));
-    if ((rc = build_device(c, c->d_P, c->d_J, c->d_K, nullptr)) != PK_OK) return rc;
-    hipLaunchKernelGGL(k_accumulate, dim3(g), dim3(256), 0, s, c->d_J, c->d_J2, c->n2);
-    PK_HIP(hipGetLastError());
+    PK_HIP(hipMemcpyAsync(c->d_J2, Pb, n2, hipMemcpyHostToDevice, s));
+    if ((rc = launch_build_uhf(c, c->d_P, c->d_J2, c->d_J, c->d_K, c->d_G, sb, c->profile)) != PK_OK) return rc;
+    PK_HIP(hipEventRecord(c->ev1, s));
     if (Jt) PK_HIP(hipMemcpyAsync(Jt, c->d_J, n2, hipMemcpyDeviceToHost, s));
-    if (Kb) PK_HIP(hipMemcpyAsync(Kb, c->d_K, n2, hipMemcpyDeviceToHost, s));
+    if (Ka) PK_HIP(hipMemcpyAsync(Ka, c->d_K, n2, hipMemcpyDeviceToHost, s));
+    if (Kb) PK_HIP(hipMemcpyAsync(Kb, c->d_G, n2, hipMemcpyDeviceToHost, s));
     PK_HIP(hipStreamSynchronize(s));
     return finish_timing(c);
 }

@@ -8,7 +8,9 @@
 
 // One J/K build from the device density P into the device matrices J and K on
 // the context's stream, with caller-provided per-build scratch: three arrays
-// of `each` doubles (scaled density, J and K accumulators).  Allocates nothing
+// of `each` doubles (scaled density, J and K accumulators) followed by `each`
+// floats (FP32 copy of the density for mixed precision), 3.5 `each` doubles
+// in all.  Allocates nothing
 // and returns without synchronising.  Fails with PK_ERR_ARG when the packed
 // matrices of the context exceed `each` elements and with PK_ERR_STATE when
 // the work lists of the current settings have not been built (pk_fock_jk

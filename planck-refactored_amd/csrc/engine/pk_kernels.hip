@@ -17,6 +17,9 @@
     }                                                                                          \
     void pk_launch_jk_os32_##a##b##c##d(const pk::JKArgs &x, int g, int b_, hipStream_t s, int &n) {  \
         pk::launch_jk_os<float, a, b, c, d>(x, g, b_, s, n);                                       \
+    }                                                                                          \
+    void pk_launch_jk_os_uhf_##a##b##c##d(const pk::JKArgs &x, int g, int b_, hipStream_t s, int &n) { \
+        pk::launch_jk_os<double, a, b, c, d, 2>(x, g, b_, s, n);                                   \
     }
 
 #define PK_SKIP(a, b, c, d)

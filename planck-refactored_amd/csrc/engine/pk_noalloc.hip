@@ -2,9 +2,10 @@
 // allocate device memory (code under HIP graph capture, tight serving loops,
 // harnesses that forbid hipMalloc).  No counterpart in the reference.
 //
-// The per-build scratch (scaled density, J and K accumulators) lives in a
-// file-scope device array of fixed size, kMaxElems doubles each: 24 MiB on
-// every GPU the library runs on, held until the process ends.  Every call on a
+// The per-build scratch (scaled density, J and K accumulators, and the FP32
+// copy of the density that mixed precision reads) lives in a file-scope device
+// array of fixed size, 3.5 kMaxElems doubles: 28 MiB on every GPU the library
+// runs on, held until the process ends.  Every call on a
 // GPU shares that GPU's copy, so calls that overlap on one GPU race; each call
 // clears the accumulators it uses (fill kernels, no memset).  On the host the
 // array's name is only a stand-in: the kernels get the device address taken
@@ -24,7 +25,8 @@ constexpr int kMaxDevices = 64;
 
 }  // namespace
 
-static __device__ __align__(256) double scratch[3 * kMaxElems];
+// scaled density, J and K accumulators (doubles), FP32 density copy (floats)
+static __device__ __align__(256) double scratch[3 * kMaxElems + kMaxElems / 2];
 
 namespace {
 

@@ -43,7 +43,7 @@ class ClassStats(ctypes.Structure):
                 ("kernels", ctypes.c_int32), ("quartets", ctypes.c_int64), ("primitive_quartets", ctypes.c_int64),
                 ("model_flops", ctypes.c_double), ("ms_total", ctypes.c_double), ("builds", ctypes.c_int64),
                 ("precision", ctypes.c_int32), ("block", ctypes.c_int32), ("remap", ctypes.c_int32),
-                ("waves", ctypes.c_int64)]
+                ("waves", ctypes.c_int64), ("alt", ctypes.c_int32), ("kind", ctypes.c_int32)]
 
 
 _lib = None
