@@ -127,10 +127,11 @@ int pk_fock_jk(pk_ctx *ctx, const double *P, double *J, double *K, int memkind);
 /* G = J[P] - K[P]/2, the return value of _compute_2e_fock (base.h:57-73). */
 int pk_fock_2e(pk_ctx *ctx, const double *P, double *G, int memkind);
 /* UHF: Jt = J[Pa + Pb], Ka = K[Pa], Kb = K[Pb]; G_a = Jt - Ka, G_b = Jt - Kb
- * (_compute_2e_fock_uhf, base.h:75-93).  One pass over the integrals: each
- * quartet is computed once (FP64) and digested into J of Pa + Pb and the two
- * exchange matrices; host or device pointers; Jt, Ka or Kb may be NULL.
- * Packed arrays for batches, as pk_fock_jk.  Runs eagerly (no graph). */
+ * (_compute_2e_fock_uhf, base.h:75-93).  One build (FP64): the Obara-Saika
+ * and cooperative classes compute each quartet once and digest it into J of
+ * Pa + Pb and both exchange matrices; the per-lane Rys classes run once per
+ * spin density.  Host or device pointers; Jt, Ka or Kb may be NULL.  Packed
+ * arrays for batches, as pk_fock_jk.  Runs eagerly (no graph). */
 int pk_fock_jk_uhf(pk_ctx *ctx, const double *Pa, const double *Pb, double *Jt, double *Ka, double *Kb,
                    int memkind);
 
