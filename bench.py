@@ -221,20 +221,24 @@ def cpu_baseline(Z, X, basis, seconds, threads):
                           stride, npairs, nq, dt, threads, stride)}
 
 
-def timed_builds(eng, P, J, K, steps, warmup):
-    """W untimed builds, then K timed ones (per-class HIP events on the
-    engine's stream); returns (ms per build, class stats with ms_avg)."""
+def timed_builds(eng, P, J, K, steps, warmup, profiled=2):
+    """W untimed builds, then K timed ones as the engine runs them (class
+    kernels spread over concurrent queues); then `profiled` more builds with
+    per-class HIP events on the engine's stream, which run the class lists one
+    at a time.  Returns (ms per timed build, seconds, class stats with ms_avg)."""
     import torch
     for _ in range(max(warmup, 1)):
         eng.jk_device(P.data_ptr(), J.data_ptr(), K.data_ptr())
-    torch.cuda.synchronize()
-    eng.profile(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
         eng.jk_device(P.data_ptr(), J.data_ptr(), K.data_ptr())
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    eng.profile(True)
+    for _ in range(profiled):
+        eng.jk_device(P.data_ptr(), J.data_ptr(), K.data_ptr())
+    torch.cuda.synchronize()
     eng.profile(False)
     cls = eng.class_stats()
     for c in cls:

@@ -35,6 +35,7 @@ struct GenArgs {
     const int *__restrict__ qket;
     const PairDev *__restrict__ pairs;
     const PrimDev *__restrict__ prims;
+    const double *__restrict__ hz;   // 1/(2 zeta) per primitive pair
     const double *__restrict__ P;    // density of J, scaled
     const double *__restrict__ PK;   // exchange density of B (RHF: P)
     const double *__restrict__ PK2;  // exchange density of B2 (k_generic<2>, one-pass UHF)
@@ -221,8 +222,10 @@ __global__ __launch_bounds__(64) void k_generic(GenArgs a) {
         __syncthreads();
         for (int kq = 0; kq < pk.nprim; ++kq) {
             const PrimDev q = a.prims[pk.prim_off + kq];
+            const double hq = a.hz[pk.prim_off + kq];
             for (int kp = 0; kp < pb.nprim; ++kp) {
                 const PrimDev p = a.prims[pb.prim_off + kp];
+                const double hp = a.hz[pb.prim_off + kp];
                 const double rs = rsqrt_nr(p.zeta + q.zeta);
                 const double inv_de = rs * rs;
                 const double rho = p.zeta * q.zeta * inv_de;
@@ -242,8 +245,8 @@ __global__ __launch_bounds__(64) void k_generic(GenArgs a) {
                     const int r = lane / 3, ax = lane - 3 * r;
                     const double ur = su[r];
                     const double B00 = 0.5 * inv_de * ur;
-                    const double B10 = (0.5 / p.zeta) * (1.0 - rho / p.zeta * ur);
-                    const double B01 = (0.5 / q.zeta) * (1.0 - rho / q.zeta * ur);
+                    const double B10 = hp * (1.0 - q.zeta * inv_de * ur);  // rho / zeta = eta / (zeta + eta)
+                    const double B01 = hq * (1.0 - p.zeta * inv_de * ur);
                     const double PQa = sel3(ax, PQ[0], PQ[1], PQ[2]);
                     const double C00 = fma(ur * (-q.zeta * inv_de), PQa, sel3(ax, p.pa[0], p.pa[1], p.pa[2]));
                     const double D00 = fma(ur * (p.zeta * inv_de), PQa, sel3(ax, q.pa[0], q.pa[1], q.pa[2]));

@@ -34,6 +34,7 @@ struct JKArgs {
     const int *__restrict__ kets;
     const PairDev *__restrict__ pairs;
     const PrimDev *__restrict__ prims;
+    const double *__restrict__ hz;  // 1/(2 zeta) per primitive pair (beside prims)
     const long long *__restrict__ mol_off;
     const int *__restrict__ mol_nb;
     const double *__restrict__ P;    // density of J (RHF: also of K), scaled by component norms
@@ -71,16 +72,17 @@ __device__ __forceinline__ double wave_sum(double v) {
 // Contracted Rys intermediates [e0|cd) of the chunk [C0, C1): acc[NE][NCH].
 template <typename R, int LA, int LB, int LC, int LD, int C0, int C1>
 __device__ __forceinline__ void rys_contract(const PairDev &pb, const PairDev &pk, int nq, int np,
-                                             const PrimDev *__restrict__ prims,
+                                             const PrimDev *__restrict__ prims, const double *__restrict__ hz,
                                              R (&acc)[Cls<LA, LB, LC, LD>::NE][C1 - C0]) {
     using K = Cls<LA, LB, LC, LD>;
     sfor<0, K::NE>([&](auto i_) { sfor<0, C1 - C0>([&](auto j_) { acc[PK_CI(i_)][PK_CI(j_)] = R(0); }); });
     const R cdv[3] = {(R)pk.ab[0], (R)pk.ab[1], (R)pk.ab[2]};
     for (int kq = 0; kq < nq; ++kq) {
         const PrimDev q = prims[pk.prim_off + kq];
+        const double hq = hz[pk.prim_off + kq];
         for (int kp = 0; kp < np; ++kp) {
             const PrimDev p = prims[pb.prim_off + kp];
-            rys_prim<R, LA, LB, LC, LD, C0, C1>(p, q, cdv, acc);
+            rys_prim<R, LA, LB, LC, LD, C0, C1>(p, q, hz[pb.prim_off + kp], hq, cdv, acc);
         }
     }
 }
@@ -88,7 +90,7 @@ __device__ __forceinline__ void rys_contract(const PairDev &pb, const PairDev &p
 // Contracted (ab|cd) of all ket components by OS/HGP: V[NAB][NCD].
 template <typename R, int LA, int LB, int LC, int LD>
 __device__ __forceinline__ void os_values(const PairDev &pb, const PairDev &pk, int nq, int np,
-                                          const PrimDev *__restrict__ prims,
+                                          const PrimDev *__restrict__ prims, const double *__restrict__ hz,
                                           R (&V)[Cls<LA, LB, LC, LD>::NAB][Cls<LA, LB, LC, LD>::NCD]) {
     using K = Cls<LA, LB, LC, LD>;
     constexpr int NE = K::NE, NF = K::NF, NAB = K::NAB;
@@ -96,9 +98,10 @@ __device__ __forceinline__ void os_values(const PairDev &pb, const PairDev &pk, 
     sfor<0, NE>([&](auto i_) { sfor<0, NF>([&](auto j_) { acc[PK_CI(i_)][PK_CI(j_)] = R(0); }); });
     for (int kq = 0; kq < nq; ++kq) {
         const PrimDev q = prims[pk.prim_off + kq];
+        const double hq = hz[pk.prim_off + kq];
         for (int kp = 0; kp < np; ++kp) {
             const PrimDev p = prims[pb.prim_off + kp];
-            os_prim<R, LA, LB, LC, LD>(p, q, acc);
+            os_prim<R, LA, LB, LC, LD>(p, q, hz[pb.prim_off + kp], hq, acc);
         }
     }
     // A->B per ket column f, then C->D per bra component ab
@@ -253,7 +256,7 @@ __global__ __launch_bounds__(256) void k_jk(JKArgs a) {
     if constexpr (ALG == ALG_OS) {
         static_assert(C0 == 0 && C1 == K::NCD, "the OS kernels take all ket components");
         R V[NAB][K::NCD];
-        os_values<R, LA, LB, LC, LD>(pb, pk, nq, pb.nprim, a.prims, V);
+        os_values<R, LA, LB, LC, LD>(pb, pk, nq, pb.nprim, a.prims, a.hz, V);
         sfor<0, NCH>([&](auto jj_) {
             R v[NAB];
             sfor<0, NAB>([&](auto i_) { v[PK_CI(i_)] = V[PK_CI(i_)][PK_CI(jj_)]; });
@@ -261,7 +264,7 @@ __global__ __launch_bounds__(256) void k_jk(JKArgs a) {
         });
     } else {
         R acc[NE][NCH];
-        rys_contract<R, LA, LB, LC, LD, C0, C1>(pb, pk, nq, pb.nprim, a.prims, acc);
+        rys_contract<R, LA, LB, LC, LD, C0, C1>(pb, pk, nq, pb.nprim, a.prims, a.hz, acc);
         sfor<0, NCH>([&](auto jj_) {
             R col[NE], v[NAB];
             sfor<0, NE>([&](auto i_) { col[PK_CI(i_)] = acc[PK_CI(i_)][PK_CI(jj_)]; });

@@ -40,20 +40,28 @@ __device__ __forceinline__ double rsqrt_nr(double x) {
 __device__ __forceinline__ double rsq(double x) { return rsqrt_nr(x); }
 __device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 
-// scalars of one primitive quartet (p on the bra, q on the ket)
+// scalars of one primitive quartet (p on the bra, q on the ket).  hz = 1/(2 zeta)
+// and he = 1/(2 eta) come from the pairs' stored reciprocals; rho/zeta =
+// eta/(zeta+eta) and rho/eta = zeta/(zeta+eta) are products, so a primitive
+// quartet divides nothing (an FP64 division is a ~10-instruction sequence).
 template <typename R>
 struct PrimQ {
     R zeta, eta, rs, inv_de, rho, PQ[3], T, pref;
+    R hz, he, roz, roe;
 };
 
 template <typename R>
-__device__ __forceinline__ PrimQ<R> prim_quartet(const PrimDev &p, const PrimDev &q) {
+__device__ __forceinline__ PrimQ<R> prim_quartet(const PrimDev &p, const PrimDev &q, double hzp, double hzq) {
     PrimQ<R> s;
     s.zeta = (R)p.zeta;
     s.eta = (R)q.zeta;
     s.rs = rsq(s.zeta + s.eta);
     s.inv_de = s.rs * s.rs;
     s.rho = s.zeta * s.eta * s.inv_de;
+    s.hz = (R)hzp;
+    s.he = (R)hzq;
+    s.roz = s.eta * s.inv_de;
+    s.roe = s.zeta * s.inv_de;
     s.PQ[0] = (R)(p.p[0] - q.p[0]);
     s.PQ[1] = (R)(p.p[1] - q.p[1]);
     s.PQ[2] = (R)(p.p[2] - q.p[2]);
@@ -249,12 +257,12 @@ __host__ __device__ constexpr int flat0_shift(int l, int k, int q, int s) {
 // |f| <= F (flat level-0 indices): bra side first, then the ket side with the
 // bra-ket coupling term (Obara & Saika 1986, Head-Gordon & Pople 1988).
 template <typename R, int LA, int LB, int LC, int LD>
-__device__ __forceinline__ void os_prim(const PrimDev &p, const PrimDev &q,
+__device__ __forceinline__ void os_prim(const PrimDev &p, const PrimDev &q, double hzp, double hzq,
                                         R (&acc)[Cls<LA, LB, LC, LD>::NE][Cls<LA, LB, LC, LD>::NF]) {
     using K = Cls<LA, LB, LC, LD>;
     constexpr int E = K::E, F = K::F, L = K::L;
     constexpr int NLE = nlev(0, E), NLF = nlev(0, F);
-    const PrimQ<R> s = prim_quartet<R>(p, q);
+    const PrimQ<R> s = prim_quartet<R>(p, q, hzp, hzq);
     const R zeta = s.zeta, eta = s.eta, inv_de = s.inv_de, rho = s.rho;
     const R PQ[3] = {s.PQ[0], s.PQ[1], s.PQ[2]};
     R Fm[L + 1];
@@ -262,7 +270,7 @@ __device__ __forceinline__ void os_prim(const PrimDev &p, const PrimDev &q,
     R V[NLE][NLF][L + 1];
     sfor<0, L + 1>([&](auto m_) { V[0][0][PK_CI(m_)] = s.pref * Fm[PK_CI(m_)]; });
     const R fp = -eta * inv_de, fq = zeta * inv_de;  // W - P = fp PQ, W - Q = fq PQ
-    const R i2z = R(0.5) / zeta, roz = rho / zeta, i2e = R(0.5) / eta, roe = rho / eta, i2d = R(0.5) * inv_de;
+    const R i2z = s.hz, roz = s.roz, i2e = s.he, roe = s.roe, i2d = R(0.5) * inv_de;
     sfor<1, E + 1>([&](auto le_) {
         constexpr int le = PK_CI(le_);
         sfor<0, ncart(le)>([&](auto k_) {
@@ -372,15 +380,15 @@ __device__ __forceinline__ void axis_2d(R seed, R C00, R D00, R B00, R B10, R B0
 // One primitive quartet by Rys quadrature: accumulate [e0|cd) of the ket
 // components [C0, C1) into acc[NE][C1 - C0]; one root per loop trip.
 template <typename R, int LA, int LB, int LC, int LD, int C0, int C1>
-__device__ __forceinline__ void rys_prim(const PrimDev &p, const PrimDev &q, const R cd[3],
+__device__ __forceinline__ void rys_prim(const PrimDev &p, const PrimDev &q, double hzp, double hzq, const R cd[3],
                                          R (&acc)[Cls<LA, LB, LC, LD>::NE][C1 - C0]) {
     using K = Cls<LA, LB, LC, LD>;
     constexpr int NR = K::NR, E = K::E;
-    const PrimQ<R> s = prim_quartet<R>(p, q);
+    const PrimQ<R> s = prim_quartet<R>(p, q, hzp, hzq);
     const R zeta = s.zeta, eta = s.eta, inv_de = s.inv_de, rho = s.rho;
     const R PQx = s.PQ[0], PQy = s.PQ[1], PQz = s.PQ[2];
     const R fp = -eta * inv_de, fq = zeta * inv_de;
-    const R i2z = R(0.5) / zeta, i2e = R(0.5) / eta, roz = rho / zeta, roe = rho / eta;
+    const R i2z = s.hz, i2e = s.he, roz = s.roz, roe = s.roe;
     const RysCtx<R> rk = rys_ctx<R, NR>(s.T);
 #pragma unroll 1
     for (int r = 0; r < NR; ++r) {

@@ -56,6 +56,11 @@ int fail(int code, const std::string &msg) {
 
 constexpr int kMaxL = PK_MAX_L;
 constexpr double kPrimCut = 1e-17;  // primitive-pair screening bound (see add_molecule)
+// Class kernels of one build run on the context stream and up to kSideQueues
+// more streams when the build holds at least kSideMinQuartets shell quartets
+// (GPU_MAX_HW_QUEUES is 4: one queue per stream).
+constexpr int kSideQueues = 3;
+constexpr int64_t kSideMinQuartets = 1 << 20;
 constexpr int kPairClasses = (kMaxL + 1) * (kMaxL + 2) / 2;
 constexpr int kQuartClasses = kPairClasses * (kPairClasses + 1) / 2;
 
@@ -234,6 +239,7 @@ struct pk_ctx {
     std::vector<double> pair_q;
     std::vector<pk::PrimDev> prims;
     std::vector<double> palpha;  // exponent on the pair's first shell, per primitive pair
+    std::vector<double> hz;      // 1/(2 zeta) per primitive pair
     std::vector<int> pair_sh;    // first and second shell of each pair
     std::vector<double> Q;  // packed nb x nb blocks, normalised functions, internal order
     pk::PairDev *d_pairs = nullptr;
@@ -241,6 +247,7 @@ struct pk_ctx {
     double *d_Q = nullptr, *d_fnorm = nullptr, *d_P = nullptr, *d_Pn = nullptr, *d_A = nullptr, *d_B = nullptr;
     double *d_J = nullptr, *d_K = nullptr;
     double *d_J2 = nullptr;  // UHF: J of the second density
+    double *d_hz = nullptr;
     double *d_palpha = nullptr, *d_Pk1n = nullptr, *d_Pk2n = nullptr, *d_gsh = nullptr;  // gradient, UHF
     double *d_B2 = nullptr;  // one-pass UHF: second exchange accumulator
     float *d_P32 = nullptr;  // mixed precision: FP32 copy of the scaled density
@@ -254,6 +261,11 @@ struct pk_ctx {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // side queues of a build: the class kernels are spread over the context
+    // stream and these (default priority), forked and joined by events
+    hipStream_t side[kSideQueues] = {};
+    hipEvent_t fork_ev = nullptr, join_ev[kSideQueues] = {};
+    bool side_ok = false;
     bool profile = false;
     pk_stats st{};
 };
@@ -333,6 +345,7 @@ int run_eri_class(pk_ctx *c, int cb, int ck, const std::vector<int> &bra, const 
     a.qket = d_k;
     a.pairs = c->d_pairs;
     a.prims = c->d_prims;
+    a.hz = c->d_hz;
     a.out = d_o;
     a.nitems = nq;
     a.la = f.la;
@@ -409,6 +422,8 @@ void parallel_for(size_t n, F f) {
 // molecules form one list per class.  In mixed precision the kets with
 // qmax_bra * qmax_ket < cutoff form a second, FP32 list of the class (classes
 // of the cooperative kernel stay FP64).
+int ensure_side(pk_ctx *c);
+
 int build_work(pk_ctx *c) {
     free_work(c);
     const int np = (int)c->pairs.size();
@@ -541,6 +556,10 @@ int build_work(pk_ctx *c) {
                 w.alt = t.alt;
             }
     PK_HIP(hipStreamSynchronize(c->stream));
+    if (c->st.quartets >= kSideMinQuartets) {
+        int rc;
+        if ((rc = ensure_side(c)) != PK_OK) return rc;
+    }
     c->work_ok = true;
     return PK_OK;
 }
@@ -578,10 +597,61 @@ BuildBufs ctx_bufs(pk_ctx *c) { return BuildBufs{c->d_Pn, c->d_A, c->d_B, c->d_P
 // sb.Pn, the accumulators cleared); nl counts the launches.  With sb.B2 set
 // (one-pass UHF) every list runs FP64 two-density digestion: the per-lane OS
 // kernels where the class has them, else the cooperative kernel.
+int launch_list(pk_ctx *c, ClassWork &w, const BuildBufs &sb, bool events, hipStream_t s, int &nl);
+
+// Side streams and events of a context, created on first use.
+int ensure_side(pk_ctx *c) {
+    if (c->side_ok) return PK_OK;
+    if (!c->fork_ev) PK_HIP(hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming));
+    for (int i = 0; i < kSideQueues; ++i) {
+        if (!c->side[i]) PK_HIP(hipStreamCreateWithFlags(&c->side[i], hipStreamNonBlocking));
+        if (!c->join_ev[i]) PK_HIP(hipEventCreateWithFlags(&c->join_ev[i], hipEventDisableTiming));
+    }
+    c->side_ok = true;
+    return PK_OK;
+}
+
 int launch_classes(pk_ctx *c, const BuildBufs &sb, bool events, int &nl) {
     int rc;
     hipStream_t s = c->stream;
-    for (ClassWork &w : c->work) {
+    // Concurrent queues (not while profiling: the class events time one list
+    // at a time): lists in descending model cost, each to the queue with the
+    // least cost so far, so that the small classes fill the tails of the
+    // large ones.  All lists add into the same accumulators with atomics.
+    if (!events && c->side_ok && c->st.quartets >= kSideMinQuartets && c->work.size() > 1) {
+        PK_HIP(hipEventRecord(c->fork_ev, s));
+        hipStream_t q[kSideQueues + 1] = {s};
+        for (int i = 0; i < kSideQueues; ++i) {
+            q[i + 1] = c->side[i];
+            PK_HIP(hipStreamWaitEvent(c->side[i], c->fork_ev, 0));
+        }
+        std::vector<size_t> order(c->work.size());
+        for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](size_t x, size_t y) { return c->work[x].flops > c->work[y].flops; });
+        double load[kSideQueues + 1] = {};
+        for (size_t i : order) {
+            int k = 0;
+            for (int j = 1; j <= kSideQueues; ++j)
+                if (load[j] < load[k]) k = j;
+            load[k] += c->work[i].flops;
+            if ((rc = launch_list(c, c->work[i], sb, false, q[k], nl)) != PK_OK) return rc;
+        }
+        for (int i = 0; i < kSideQueues; ++i) {
+            PK_HIP(hipEventRecord(c->join_ev[i], c->side[i]));
+            PK_HIP(hipStreamWaitEvent(s, c->join_ev[i], 0));
+        }
+        return PK_OK;
+    }
+    for (ClassWork &w : c->work)
+        if ((rc = launch_list(c, w, sb, events, s, nl)) != PK_OK) return rc;
+    return PK_OK;
+}
+
+// Enqueue the kernels of one class list on stream s.
+int launch_list(pk_ctx *c, ClassWork &w, const BuildBufs &sb, bool events, hipStream_t s, int &nl) {
+    int rc;
+    {
         const int *wv = w.d_wave;
         pk::JKArgs a{};
         a.wave_bra = wv;
@@ -591,6 +661,7 @@ int launch_classes(pk_ctx *c, const BuildBufs &sb, bool events, int &nl) {
         a.kets = w.d_kets;
         a.pairs = c->d_pairs;
         a.prims = c->d_prims;
+        a.hz = c->d_hz;
         a.mol_off = c->d_mol_off;
         a.mol_nb = c->d_mol_nb;
         a.P = sb.Pn;
@@ -646,6 +717,7 @@ int launch_classes(pk_ctx *c, const BuildBufs &sb, bool events, int &nl) {
             g.mol_nb = a.mol_nb;
             g.pairs = a.pairs;
             g.prims = a.prims;
+            g.hz = a.hz;
             g.P = a.P;
             g.PK = uhf ? a.PK : a.P;
             g.PK2 = a.PK2;
@@ -953,6 +1025,7 @@ int add_molecule(pk_ctx *c, const pk_basis_desc *d, int m) {
             if (std::fabs(pp_all[k][i].kc) * kmax >= kPrimCut) {
                 c->prims.push_back(pp_all[k][i]);
                 c->palpha.push_back(pa_all[k][i]);
+                c->hz.push_back(0.5 / pp_all[k][i].zeta);
             }
         p.nprim = (int)c->prims.size() - p.prim_off;
         p.grp = prim_group(p.nprim);
@@ -995,6 +1068,7 @@ int ctx_build(const pk_basis_desc *descs, int nmol, pk_ctx **out) {
         (rc = dev_upload(&c->d_pairs, c->pairs, c->stream)) != PK_OK ||
         (rc = dev_upload(&c->d_prims, c->prims, c->stream)) != PK_OK ||
         (rc = dev_upload(&c->d_palpha, c->palpha, c->stream)) != PK_OK ||
+        (rc = dev_upload(&c->d_hz, c->hz, c->stream)) != PK_OK ||
         (rc = dev_upload(&c->d_pair_sh, c->pair_sh, c->stream)) != PK_OK ||
         (rc = dev_upload(&c->d_fnorm, c->fnorm, c->stream)) != PK_OK ||
         (rc = dev_upload(&c->d_fpos, c->fpos, c->stream)) != PK_OK || (rc = build_schwarz(c)) != PK_OK ||
@@ -1111,11 +1185,16 @@ int pk_ctx_destroy(pk_ctx *c) {
     free_work(c);
     for (void *p : {(void *)c->d_pairs, (void *)c->d_prims, (void *)c->d_Q, (void *)c->d_fnorm, (void *)c->d_P,
                     (void *)c->d_Pn, (void *)c->d_A, (void *)c->d_B, (void *)c->d_J, (void *)c->d_K, (void *)c->d_G, (void *)c->d_J2,
-                    (void *)c->d_palpha, (void *)c->d_B2, (void *)c->d_P32, (void *)c->d_Pk1n, (void *)c->d_Pk2n, (void *)c->d_gsh, (void *)c->d_pair_sh,
+                    (void *)c->d_palpha, (void *)c->d_hz, (void *)c->d_B2, (void *)c->d_P32, (void *)c->d_Pk1n, (void *)c->d_Pk2n, (void *)c->d_gsh, (void *)c->d_pair_sh,
                     (void *)c->d_mol_off, (void *)c->d_mol_nb, (void *)c->d_mol_foff, (void *)c->d_fpos})
         if (p) (void)hipFree(p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
+    for (int i = 0; i < kSideQueues; ++i) {
+        if (c->join_ev[i]) (void)hipEventDestroy(c->join_ev[i]);
+        if (c->side[i]) (void)hipStreamDestroy(c->side[i]);
+    }
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return PK_OK;
@@ -1314,6 +1393,7 @@ int pk_eri_gradient(pk_ctx *c, const double *Pj, const double *Pk1, const double
         a.pairs = c->d_pairs;
         a.prims = c->d_prims;
         a.palpha = c->d_palpha;
+        a.hz = c->d_hz;
         a.pair_sh = c->d_pair_sh;
         a.Pj = c->d_Pn;
         a.Pk1 = c->d_Pk1n;
