@@ -95,6 +95,11 @@ constexpr bool os_available(int la, int lb, int lc, int ld) {
     return la + lb + lc + ld <= 4 && nlev(la, la + lb) * nlev(lc, lc + ld) <= 54;
 }
 
+// dynamic LDS of a k_jk launch: the exchange rows of segment mode
+inline size_t jk_lds_bytes(const JKArgs &a, int la, int lb, int nk) {
+    return a.segmode ? ((size_t)nk * (ncart(la) + ncart(lb)) * a.kcols + ncart(la) * ncart(lb)) * sizeof(double) : 0;
+}
+
 // grid x block (block 64, 128 or 256 lanes = 1, 2 or 4 waves)
 template <typename R, int LA, int LB, int LC, int LD, int C0>
 void launch_jk_rys(const JKArgs &a, int grid, int block, hipStream_t s, int &nlaunch) {
@@ -102,7 +107,8 @@ void launch_jk_rys(const JKArgs &a, int grid, int block, hipStream_t s, int &nla
     if constexpr (rys_tmpl_available(LA, LB, LC, LD) && C0 < NCD) {
         constexpr int CH = chunk_size(LA, LB, LC, LD);
         constexpr int C1 = C0 + CH < NCD ? C0 + CH : NCD;
-        hipLaunchKernelGGL((k_jk<R, LA, LB, LC, LD, ALG_RYS, C0, C1>), dim3(grid), dim3(block), 0, s, a);
+        hipLaunchKernelGGL((k_jk<R, LA, LB, LC, LD, ALG_RYS, C0, C1>), dim3(grid), dim3(block),
+                           jk_lds_bytes(a, LA, LB, 1), s, a);
         ++nlaunch;
         launch_jk_rys<R, LA, LB, LC, LD, C1>(a, grid, block, s, nlaunch);
     }
@@ -112,7 +118,8 @@ template <typename R, int LA, int LB, int LC, int LD, int NK = 1>
 void launch_jk_os(const JKArgs &a, int grid, int block, hipStream_t s, int &nlaunch) {
     if constexpr (os_available(LA, LB, LC, LD)) {
         constexpr int NCD = ncart(LC) * ncart(LD);
-        hipLaunchKernelGGL((k_jk<R, LA, LB, LC, LD, ALG_OS, 0, NCD, NK>), dim3(grid), dim3(block), 0, s, a);
+        hipLaunchKernelGGL((k_jk<R, LA, LB, LC, LD, ALG_OS, 0, NCD, NK>), dim3(grid), dim3(block),
+                           jk_lds_bytes(a, LA, LB, NK), s, a);
         ++nlaunch;
     }
 }

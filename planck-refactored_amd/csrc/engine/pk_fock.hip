@@ -17,6 +17,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <numeric>
@@ -61,6 +62,10 @@ constexpr double kPrimCut = 1e-17;  // primitive-pair screening bound (see add_m
 // (GPU_MAX_HW_QUEUES is 4: one queue per stream).
 constexpr int kSideQueues = 3;
 constexpr int64_t kSideMinQuartets = 1 << 20;
+// Segment mode of the per-lane kernels (see k_jk): at most kSegWaves waves of
+// one bra per segment, and at most kSegLdsMax bytes of LDS exchange rows.
+constexpr int kSegWaves = 16;
+constexpr size_t kSegLdsMax = 64 * 1024;
 constexpr int kPairClasses = (kMaxL + 1) * (kMaxL + 2) / 2;
 constexpr int kQuartClasses = kPairClasses * (kPairClasses + 1) / 2;
 
@@ -68,6 +73,7 @@ struct ClassFns {
     int la, lb, lc, ld;
     bool has_os;    // Obara-Saika per-lane kernel
     bool has_tmpl;  // Rys per-lane kernel (else the cooperative kernel)
+    bool seg;       // the per-lane kernels run in segment mode (pk::seg_class)
     void (*jk_os)(const pk::JKArgs &, int, int, hipStream_t, int &);
     void (*jk_rys)(const pk::JKArgs &, int, int, hipStream_t, int &);
     void (*jk_os32)(const pk::JKArgs &, int, int, hipStream_t, int &);  // FP32 integrals
@@ -77,6 +83,7 @@ struct ClassFns {
 
 #define PK_ENTRY(a, b, c, d, shard)                                                             \
     ClassFns{a, b, c, d, pk::os_available(a, b, c, d), pk::rys_tmpl_available(a, b, c, d),      \
+             pk::seg_class(a, b, c, d),                                                         \
              &pk_launch_jk_os_##a##b##c##d, &pk_launch_jk_rys_##a##b##c##d,                     \
              &pk_launch_jk_os32_##a##b##c##d, &pk_launch_jk_rys32_##a##b##c##d,                 \
              &pk_launch_jk_os_uhf_##a##b##c##d},
@@ -191,6 +198,9 @@ struct ClassWork {
     double flops = 0.0;
     int *d_wave = nullptr;  // [4][nwaves]: bra, start, count, molecule
     int *d_kets = nullptr;
+    int nseg = 0;          // segments: runs of at most kSegWaves waves of one bra
+    int *d_seg = nullptr;  // [nseg + 1] first wave of each segment
+    int kcols = 0;         // LDS columns of an exchange row (largest molecule)
     int kernels = 0;
     double ms_total = 0.0;
     int64_t builds = 0;
@@ -217,6 +227,9 @@ struct pk_ctx {
     int nb = 0, ns = 0;  // functions and shells over all molecules
     int nmol = 1;
     std::vector<int> mol_nb, mol_foff;  // functions per molecule, first function
+    std::vector<int> tblk;              // [5 * nmol]: first function of type s, p, d, f (internal order), nb
+    int *d_tblk = nullptr;
+    bool seg_on = true;                 // segment mode of the per-lane kernels (PK_EXCHANGE_ROWS=0: off)
     std::vector<long long> mol_off;     // offset of the molecule's nb x nb block
     size_t n2 = 0;                      // packed matrix length
     int nbmax = 0;
@@ -297,6 +310,7 @@ void free_work(pk_ctx *c) {
     for (ClassWork &w : c->work) {
         if (w.d_wave) (void)hipFree(w.d_wave);
         if (w.d_kets) (void)hipFree(w.d_kets);
+        if (w.d_seg) (void)hipFree(w.d_seg);
         if (w.e0) (void)hipEventDestroy(w.e0);
         if (w.e1) (void)hipEventDestroy(w.e1);
     }
@@ -539,6 +553,23 @@ int build_work(pk_ctx *c) {
                 int rc;
                 if ((rc = dev_upload(&w.d_wave, packed, c->stream)) != PK_OK) return rc;
                 if ((rc = dev_upload(&w.d_kets, kets.get(), koff.back(), c->stream)) != PK_OK) return rc;
+                if (f.has_tmpl) {
+                    // segments of the per-lane kernels: each bra's waves in runs
+                    // of at most kSegWaves; LDS columns: the two ket shell types
+                    // of the largest molecule
+                    std::vector<int> seg;
+                    for (size_t t = 0; t < tasks.size(); ++t) {
+                        for (size_t w0 = woff[t]; w0 < woff[t + 1]; w0 += kSegWaves) seg.push_back((int)w0);
+                        if (woff[t + 1] > woff[t]) {
+                            const int *tb = &c->tblk[5 * tasks[t].first];
+                            const int ncol = tb[f.lc + 1] - tb[f.lc] + (f.lc == f.ld ? 0 : tb[f.ld + 1] - tb[f.ld]);
+                            w.kcols = std::max(w.kcols, ncol);
+                        }
+                    }
+                    seg.push_back((int)nw);
+                    w.nseg = (int)seg.size() - 1;
+                    if ((rc = dev_upload(&w.d_seg, seg, c->stream)) != PK_OK) return rc;
+                }
                 c->st.quartets += w.nquart;
                 c->st.primitive_quartets += w.nprimq;
                 c->st.model_flops += w.flops;
@@ -675,6 +706,10 @@ int launch_list(pk_ctx *c, ClassWork &w, const BuildBufs &sb, bool events, hipSt
         a.tol = c->tol;
         a.nwaves = w.nwaves;
         a.remap = w.remap;
+        a.seg = w.d_seg;
+        a.tblk = c->d_tblk;
+        a.nseg = w.nseg;
+        a.kcols = w.kcols;
         const bool uhf = sb.B2 != nullptr;
         if (events) {
             if (!w.e0) PK_HIP(hipEventCreate(&w.e0));
@@ -685,7 +720,15 @@ int launch_list(pk_ctx *c, ClassWork &w, const BuildBufs &sb, bool events, hipSt
         const ClassFns &f = kClasses[w.cls];
         const bool fp32 = w.prec == PK_PREC_MIXED && !w.alt;
         const int wpb = w.block / 64;
-        int grid = (w.nwaves + wpb - 1) / wpb;
+        // segment mode when the exchange rows of the call fit the LDS budget
+        // (two rows sets for the one-pass UHF OS kernels)
+        {
+            const int nk = uhf && use_os(c, f) ? 2 : 1;
+            const size_t rows = (size_t)nk * (pk::ncart(f.la) + pk::ncart(f.lb)) * w.kcols +
+                                (size_t)pk::ncart(f.la) * pk::ncart(f.lb);
+            a.segmode = c->seg_on && f.seg && w.nseg > 0 && rows * sizeof(double) <= kSegLdsMax;
+        }
+        int grid = a.segmode ? w.nseg : (w.nwaves + wpb - 1) / wpb;
         if (w.remap) grid = (grid + 7) / 8 * 8;
         if (uhf && use_os(c, f)) {
             f.jk_os_uhf(a, grid, w.block, s, nl);
@@ -908,12 +951,15 @@ int add_molecule(pk_ctx *c, const pk_basis_desc *d, int m) {
     std::vector<int> spos(d->nshells), snat(d->nshells);
     {
         int pos = 0;
-        for (int l = 0; l <= kMaxL; ++l)
+        for (int l = 0; l <= kMaxL; ++l) {
+            c->tblk.push_back(pos);
             for (int s = 0; s < d->nshells; ++s)
                 if (d->l[s] == l) {
                     spos[s] = pos;
                     pos += pk::ncart(l);
                 }
+        }
+        c->tblk.push_back(pos);
         for (int s = 0, f = 0; s < d->nshells; ++s) {
             snat[s] = f;
             f += pk::ncart(d->l[s]);
@@ -1054,6 +1100,9 @@ int ctx_build(const pk_basis_desc *descs, int nmol, pk_ctx **out) {
     }
     c->st.nbasis = c->nb;
     c->st.nshells = c->ns;
+    // diagnostic switch: PK_EXCHANGE_ROWS=0 runs the per-lane kernels without
+    // the LDS exchange rows (one global atomic per exchange contribution)
+    if (const char *e = std::getenv("PK_EXCHANGE_ROWS")) c->seg_on = std::atoi(e) != 0;
     c->st.shell_pairs = (int64_t)c->pairs.size();
     int rc = PK_OK;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
@@ -1065,6 +1114,7 @@ int ctx_build(const pk_basis_desc *descs, int nmol, pk_ctx **out) {
     if ((rc = dev_upload(&c->d_mol_off, c->mol_off, c->stream)) != PK_OK ||
         (rc = dev_upload(&c->d_mol_nb, c->mol_nb, c->stream)) != PK_OK ||
         (rc = dev_upload(&c->d_mol_foff, c->mol_foff, c->stream)) != PK_OK ||
+        (rc = dev_upload(&c->d_tblk, c->tblk, c->stream)) != PK_OK ||
         (rc = dev_upload(&c->d_pairs, c->pairs, c->stream)) != PK_OK ||
         (rc = dev_upload(&c->d_prims, c->prims, c->stream)) != PK_OK ||
         (rc = dev_upload(&c->d_palpha, c->palpha, c->stream)) != PK_OK ||
@@ -1186,7 +1236,7 @@ int pk_ctx_destroy(pk_ctx *c) {
     for (void *p : {(void *)c->d_pairs, (void *)c->d_prims, (void *)c->d_Q, (void *)c->d_fnorm, (void *)c->d_P,
                     (void *)c->d_Pn, (void *)c->d_A, (void *)c->d_B, (void *)c->d_J, (void *)c->d_K, (void *)c->d_G, (void *)c->d_J2,
                     (void *)c->d_palpha, (void *)c->d_hz, (void *)c->d_B2, (void *)c->d_P32, (void *)c->d_Pk1n, (void *)c->d_Pk2n, (void *)c->d_gsh, (void *)c->d_pair_sh,
-                    (void *)c->d_mol_off, (void *)c->d_mol_nb, (void *)c->d_mol_foff, (void *)c->d_fpos})
+                    (void *)c->d_mol_off, (void *)c->d_mol_nb, (void *)c->d_mol_foff, (void *)c->d_tblk, (void *)c->d_fpos})
         if (p) (void)hipFree(p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
