@@ -114,8 +114,10 @@ struct GenTabs {
 };
 
 // NK = 2: one-pass UHF digestion (J of P into A, exchange of PK into B and of
-// PK2 into B2).
-template <int NK>
+// PK2 into B2).  TLA..TLD >= 0 fix the class at compile time (every size,
+// loop bound and index division below is then a constant); -1 takes it from
+// the arguments (one kernel for every class: ERI output, Schwarz table).
+template <int NK, int TLA = -1, int TLB = -1, int TLC = -1, int TLD = -1>
 __global__ __launch_bounds__(64) void k_generic(GenArgs a) {
     extern __shared__ double smem[];
     __shared__ double su[GEN_NR], sw[GEN_NR];
@@ -128,7 +130,8 @@ __global__ __launch_bounds__(64) void k_generic(GenArgs a) {
     const int lane = threadIdx.x;
     const int item = blockIdx.x;
     if (item >= a.nitems) return;
-    const int la = a.la, lb = a.lb, lc = a.lc, ld = a.ld;
+    const int la = TLA >= 0 ? TLA : a.la, lb = TLB >= 0 ? TLB : a.lb;
+    const int lc = TLC >= 0 ? TLC : a.lc, ld = TLD >= 0 ? TLD : a.ld;
     const int NA = (la + 1) * (la + 2) / 2, NB = (lb + 1) * (lb + 2) / 2;
     const int NC = (lc + 1) * (lc + 2) / 2, ND = (ld + 1) * (ld + 2) / 2;
     const int NAB = NA * NB, NCD = NC * ND, NABCD = NAB * NCD;

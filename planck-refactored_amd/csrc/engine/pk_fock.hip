@@ -323,8 +323,42 @@ void free_work(pk_ctx *c) {
 // all of class (cb, ck) with cb >= ck.  out gets NA NB NC ND values per quartet.
 // One 64-lane workgroup per item; the dynamic LDS holds the transferred 1-D
 // factors and the quartet's accumulators (at most 123 KB for (ff|ff)).
+// Cooperative kernels with the class fixed at compile time, for the J/K lists
+// of the classes without a per-lane kernel (null elsewhere).
+struct GenKernels {
+    const void *k1, *k2;  // one and two exchange densities
+};
+template <int A, int B, int C, int D>
+constexpr GenKernels gen_kernels() {
+    if constexpr (!pk::rys_tmpl_available(A, B, C, D))
+        return GenKernels{(const void *)pk::k_generic<1, A, B, C, D>, (const void *)pk::k_generic<2, A, B, C, D>};
+    else
+        return GenKernels{nullptr, nullptr};
+}
+#define PK_GEN_ENTRY(a, b, c, d, shard) gen_kernels<a, b, c, d>(),
+const GenKernels kGenKernels[kQuartClasses] = {PK_FOR_EACH_CLASS(PK_GEN_ENTRY)};
+
 int launch_generic(const pk::GenArgs &a, hipStream_t s) {
     if (a.nitems <= 0) return PK_OK;
+    if (!a.eri_mode) {
+        const int cb = pk::pair_class(a.la, a.lb), ck = pk::pair_class(a.lc, a.ld);
+        const GenKernels &g = kGenKernels[quart_class(cb, ck)];
+        const void *k = a.B2 ? g.k2 : g.k1;
+        if (k) {
+            static std::atomic<int> attr_done[2 * kQuartClasses];  // zero-initialised
+            const size_t bytes = pk::generic_lds_bytes(a.la, a.lb, a.lc, a.ld);
+            std::atomic<int> &done = attr_done[2 * quart_class(cb, ck) + (a.B2 ? 1 : 0)];
+            if (!done.load()) {
+                PK_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)pk::generic_lds_bytes(kMaxL, kMaxL, kMaxL, kMaxL)));
+                done.store(1);
+            }
+            pk::GenArgs x = a;
+            void *args[] = {&x};
+            PK_HIP(hipLaunchKernel(k, dim3(a.nitems), dim3(64), args, bytes, s));
+            return PK_OK;
+        }
+    }
     static const size_t max_bytes = pk::generic_lds_bytes(kMaxL, kMaxL, kMaxL, kMaxL);
     static const hipError_t attr1 =
         hipFuncSetAttribute((const void *)pk::k_generic<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)max_bytes);
