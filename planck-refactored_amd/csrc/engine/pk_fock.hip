@@ -279,6 +279,7 @@ struct pk_ctx {
     hipStream_t side[kSideQueues] = {};
     hipEvent_t fork_ev = nullptr, join_ev[kSideQueues] = {};
     bool side_ok = false;
+    bool side_off = false;  // PK_SIDE_QUEUES=0
     bool profile = false;
     pk_stats st{};
 };
@@ -683,7 +684,7 @@ int launch_classes(pk_ctx *c, const BuildBufs &sb, bool events, int &nl) {
     // at a time): lists in descending model cost, each to the queue with the
     // least cost so far, so that the small classes fill the tails of the
     // large ones.  All lists add into the same accumulators with atomics.
-    if (!events && c->side_ok && c->st.quartets >= kSideMinQuartets && c->work.size() > 1) {
+    if (!events && c->side_ok && !c->side_off && c->st.quartets >= kSideMinQuartets && c->work.size() > 1) {
         PK_HIP(hipEventRecord(c->fork_ev, s));
         hipStream_t q[kSideQueues + 1] = {s};
         for (int i = 0; i < kSideQueues; ++i) {
@@ -1137,6 +1138,9 @@ int ctx_build(const pk_basis_desc *descs, int nmol, pk_ctx **out) {
     // diagnostic switch: PK_EXCHANGE_ROWS=0 runs the per-lane kernels without
     // the LDS exchange rows (one global atomic per exchange contribution)
     if (const char *e = std::getenv("PK_EXCHANGE_ROWS")) c->seg_on = std::atoi(e) != 0;
+    // diagnostic switch: PK_SIDE_QUEUES=0 keeps every class kernel of a build
+    // on the context stream (one kernel at a time, as a kernel trace times them)
+    if (const char *e = std::getenv("PK_SIDE_QUEUES")) c->side_off = std::atoi(e) == 0;
     c->st.shell_pairs = (int64_t)c->pairs.size();
     int rc = PK_OK;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||

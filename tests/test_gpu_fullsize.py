@@ -107,3 +107,31 @@ def test_fullsize_mixed_jk_matches_oracle(name, cutoff):
             assert dj < bar and dk < bar
     finally:
         eng.set("os", meta["tol_eri"])
+
+
+def test_fullsize_graph_replays_with_side_queues_match_eager():
+    """n-C10H22 / cc-pVTZ holds more than 2^20 shell quartets, so a build
+    spreads its class kernels over the context stream and three side streams
+    (fork and join by events); captured into a hipGraph, the fork and join are
+    part of the graph.  Four replays, with eager builds between them, against
+    the eager build of the same density."""
+    d, meta = _fixture("c10h22_ccpvtz")
+    eng = _engine(meta)
+    eng.set("os", meta["tol_eri"])
+    P = _densities(meta)["signed"]
+    assert eng.stats()["quartets"] >= 1 << 20
+    J0, K0 = eng.jk(P)
+    try:
+        eng.set_graph(True)
+        for rep in range(4):
+            Jg, Kg = eng.jk(P)
+            eng.set_graph(False)
+            Je, Ke = eng.jk(P)  # eager work between the replays
+            eng.set_graph(True)
+            dj = max(np.abs(Jg - J0).max(), np.abs(Je - J0).max())
+            dk = max(np.abs(Kg - K0).max(), np.abs(Ke - K0).max())
+            print("replay %d: max|dJ| %.2e  max|dK| %.2e" % (rep, dj, dk))
+            # the FP64 atomics add in a different order on every build
+            assert dj < 1e-12 * max(1.0, np.abs(J0).max()) and dk < 1e-12 * max(1.0, np.abs(K0).max())
+    finally:
+        eng.set_graph(False)
