@@ -94,6 +94,24 @@ __device__ __forceinline__ void seg_reduce(int lane, int nout, int nterm, Term t
     }
 }
 
+// out[o] = sum_t acc[a_of(o) + b_of(t)] S[t] for o in [0, NOUT), one lane per
+// output and the NTERM terms unrolled: b_of(t) of a compile-time t folds to an
+// immediate LDS offset, so a term costs two LDS reads and one FMA (the class
+// of the kernel fixed at compile time).  emit(o, value) takes the sum.  NL
+// lanes share the outputs.
+template <int NOUT, int NTERM, int NL, typename AO, typename BO, typename Emit>
+__device__ __forceinline__ void contract_ct(int lane, const double *acc, const double *S, AO a_of, BO b_of, Emit emit) {
+    for (int o = lane; o < NOUT; o += NL) {
+        const double *base = acc + a_of(o);
+        double s[4] = {0.0, 0.0, 0.0, 0.0};  // four independent FMA chains
+        sfor<0, NTERM>([&](auto t_) {
+            constexpr int t = PK_CI(t_);
+            s[t & 3] = fma(base[b_of(t_)], S[t], s[t & 3]);
+        });
+        emit(o, (s[0] + s[1]) + (s[2] + s[3]));
+    }
+}
+
 // Cartesian exponents (x, y, z) of component k of a shell of angular momentum
 // l <= 3, in the order lx descending, then ly descending
 static __device__ const unsigned char kCartTab[4][10][3] = {
@@ -117,8 +135,21 @@ struct GenTabs {
 // PK2 into B2).  TLA..TLD >= 0 fix the class at compile time (every size,
 // loop bound and index division below is then a constant); -1 takes it from
 // the arguments (one kernel for every class: ERI output, Schwarz table).
-template <int NK, int TLA = -1, int TLB = -1, int TLC = -1, int TLD = -1>
-__global__ __launch_bounds__(64) void k_generic(GenArgs a) {
+// Waves per quartet of the compile-time-class kernel: two for the classes of at
+// least 1800 integrals per quartet (measured on n-C10H22 / cc-pVTZ: (dd|dp)
+// 5.3 -> 3.5 ms, (fd|fd) 3.2 -> 1.9 ms, (ff|dd) 2.2 -> 1.3 ms), one below
+// ((fd|dp) 14.0 -> 15.0 ms, (fp|fp) 3.8 -> 4.6 ms with two).
+constexpr int gen_waves(int la, int lb, int lc, int ld) {
+    return ncart(la) * ncart(lb) * ncart(lc) * ncart(ld) >= 1800 ? 2 : 1;
+}
+
+// NW waves per workgroup (NL = 64 NW lanes) share each quartet; NW > 1 needs
+// the class at compile time.
+template <int NK, int TLA = -1, int TLB = -1, int TLC = -1, int TLD = -1, int NW = 1>
+__global__ __launch_bounds__(64 * NW) void k_generic(GenArgs a) {
+    constexpr int NL = 64 * NW;
+    static_assert(NW == 1 || (TLA >= 0 && TLB >= 0 && TLC >= 0 && TLD >= 0),
+                  "several waves per quartet need the class at compile time (shuffle-free digestion)");
     extern __shared__ double smem[];
     __shared__ double su[GEN_NR], sw[GEN_NR];
     __shared__ double hab[3][10], hcd[3][10];  // C(b,k) AB^(b-k) per axis at [b(b+1)/2 + k]; ket likewise
@@ -143,23 +174,23 @@ __global__ __launch_bounds__(64) void k_generic(GenArgs a) {
     double *G = I2 + n * 3 * NI;         // [n * 3][la + 1][lb + 1][lc + 1][ld + 1]
     double *acc = G + n * 3 * NG;        // [NABCD]
 
-    for (int i = lane; i < NAB; i += 64) {
+    for (int i = lane; i < NAB; i += NL) {
         const int ia = i / NB, ib = i % NB;
         tb.ab_a[i] = ia;
         tb.ab_b[i] = ib;
         for (int q = 0; q < 3; ++q) tb.ab_g[i][q] = (kCartTab[la][ia][q] * D2 + kCartTab[lb][ib][q]) * D34;
     }
-    for (int i = lane; i < NCD; i += 64) {
+    for (int i = lane; i < NCD; i += NL) {
         const int ic = i / ND, id = i % ND;
         tb.cd_c[i] = ic;
         tb.cd_d[i] = id;
         for (int q = 0; q < 3; ++q) tb.cd_g[i][q] = kCartTab[lc][ic][q] * D4 + kCartTab[ld][id][q];
     }
-    for (int i = lane; i < NA * NC; i += 64) tb.ac_a[i] = i / NC, tb.ac_c[i] = i % NC;
-    for (int i = lane; i < NB * ND; i += 64) tb.bd_b[i] = i / ND, tb.bd_d[i] = i % ND;
-    for (int i = lane; i < NB * NC; i += 64) tb.bc_b[i] = i / NC, tb.bc_c[i] = i % NC;
-    for (int i = lane; i < NA * ND; i += 64) tb.ad_a[i] = i / ND, tb.ad_d[i] = i % ND;
-    for (int g = lane; g < NG; g += 64) {
+    for (int i = lane; i < NA * NC; i += NL) tb.ac_a[i] = i / NC, tb.ac_c[i] = i % NC;
+    for (int i = lane; i < NB * ND; i += NL) tb.bd_b[i] = i / ND, tb.bd_d[i] = i % ND;
+    for (int i = lane; i < NB * NC; i += NL) tb.bc_b[i] = i / NC, tb.bc_c[i] = i % NC;
+    for (int i = lane; i < NA * ND; i += NL) tb.ad_a[i] = i / ND, tb.ad_d[i] = i % ND;
+    for (int g = lane; g < NG; g += NL) {
         int t = g;
         tb.g_d[g] = t % D4;
         t /= D4;
@@ -171,8 +202,8 @@ __global__ __launch_bounds__(64) void k_generic(GenArgs a) {
     // this lane's first (ab, cd) in the flattened component loops, and the
     // step of 64 components in (ab, cd); likewise (root-axis, g) of the
     // transfer loop
-    const int ab0 = lane / NCD, cd0 = lane % NCD, sab = 64 / NCD, scd = 64 % NCD;
-    const int ra0 = lane / NG, g0 = lane % NG, sra = 64 / NG, sg = 64 % NG;
+    const int ab0 = lane / NCD, cd0 = lane % NCD, sab = NL / NCD, scd = NL % NCD;
+    const int ra0 = lane / NG, g0 = lane % NG, sra = NL / NG, sg = NL % NG;
     __syncthreads();
 
     // Class fixed at compile time and at most RACC_MAX components per lane
@@ -187,7 +218,7 @@ __global__ __launch_bounds__(64) void k_generic(GenArgs a) {
     constexpr int T_NABCD = TMPL ? ncart(TLA) * ncart(TLB) * ncart(TLC) * ncart(TLD) : 1;
     constexpr int T_N = TMPL ? (TLA + TLB + TLC + TLD) / 2 + 1 : 1;
     constexpr int T_NG = TMPL ? (TLA + 1) * (TLB + 1) * (TLC + 1) * (TLD + 1) : 1;
-    constexpr int KREG = (T_NABCD + 63) / 64, KTR = (3 * T_N * T_NG + 63) / 64;
+    constexpr int KREG = (T_NABCD + NL - 1) / NL, KTR = (3 * T_N * T_NG + NL - 1) / NL;
     constexpr int RACC_MAX = 15;
     constexpr bool RACC = TMPL && KREG <= RACC_MAX;
     unsigned goff[RACC ? KREG : 1], gtr[RACC ? KTR : 1];
@@ -197,7 +228,7 @@ __global__ __launch_bounds__(64) void k_generic(GenArgs a) {
         sfor<0, KREG>([&](auto k_) {
             constexpr int k = PK_CI(k_);
             unsigned v = 0;
-            if (lane + 64 * k < T_NABCD)
+            if (lane + NL * k < T_NABCD)
                 v = (unsigned)(tb.ab_g[ab][0] + tb.cd_g[cd][0]) | ((unsigned)(tb.ab_g[ab][1] + tb.cd_g[cd][1]) << 8) |
                     ((unsigned)(tb.ab_g[ab][2] + tb.cd_g[cd][2]) << 16);
             goff[k] = v;
@@ -209,7 +240,7 @@ __global__ __launch_bounds__(64) void k_generic(GenArgs a) {
         sfor<0, KTR>([&](auto j_) {
             constexpr int j = PK_CI(j_);
             unsigned v = 0xffffffffu;
-            if (lane + 64 * j < 3 * T_N * T_NG)
+            if (lane + NL * j < 3 * T_N * T_NG)
                 v = (unsigned)g | ((unsigned)ra << 8) | ((unsigned)tb.g_a[g] << 16) | ((unsigned)tb.g_b[g] << 18) |
                     ((unsigned)tb.g_c[g] << 20) | ((unsigned)tb.g_d[g] << 22);
             gtr[j] = v;
@@ -260,7 +291,7 @@ __global__ __launch_bounds__(64) void k_generic(GenArgs a) {
         if constexpr (RACC)
             sfor<0, KREG>([&](auto k_) { racc[PK_CI(k_)] = 0.0; });
         else
-            for (int i = lane; i < NABCD; i += 64) acc[i] = 0.0;
+            for (int i = lane; i < NABCD; i += NL) acc[i] = 0.0;
         if (lane < 30) {
             const int ax = lane / 10, t = lane % 10;
             const int b = t >= 6 ? 3 : (t >= 3 ? 2 : (t >= 1 ? 1 : 0)), k = t - b * (b + 1) / 2;
@@ -360,7 +391,7 @@ __global__ __launch_bounds__(64) void k_generic(GenArgs a) {
                     });
                 } else {
                     int ra = ra0, g = g0;
-                    for (int t = lane; t < 3 * n * NG; t += 64) {
+                    for (int t = lane; t < 3 * n * NG; t += NL) {
                         const int ax = ra % 3;
                         const int ia = tb.g_a[g], ib = tb.g_b[g], ic = tb.g_c[g], id = tb.g_d[g];
                         const double *I = I2 + ra * NI + ia * F1 + ic;
@@ -392,7 +423,7 @@ __global__ __launch_bounds__(64) void k_generic(GenArgs a) {
                     });
                 } else {
                     int ab = ab0, cd = cd0;
-                    for (int i = lane; i < NABCD; i += 64) {
+                    for (int i = lane; i < NABCD; i += NL) {
                         const int o0 = tb.ab_g[ab][0] + tb.cd_g[cd][0], o1 = tb.ab_g[ab][1] + tb.cd_g[cd][1],
                                   o2 = tb.ab_g[ab][2] + tb.cd_g[cd][2];
                         double s = 0.0;
@@ -411,13 +442,13 @@ __global__ __launch_bounds__(64) void k_generic(GenArgs a) {
             // the quartet's integrals into LDS for the digestion
             sfor<0, KREG>([&](auto k_) {
                 constexpr int k = PK_CI(k_);
-                if (lane + 64 * k < T_NABCD) acc[lane + 64 * k] = racc[k];
+                if (lane + NL * k < T_NABCD) acc[lane + NL * k] = racc[k];
             });
             __syncthreads();
         }
         if (a.eri_mode) {
             double *o = a.out + (size_t)item * NABCD;
-            for (int i = lane; i < NABCD; i += 64) o[i] = acc[i];
+            for (int i = lane; i < NABCD; i += NL) o[i] = acc[i];
             __syncthreads();
             continue;
         }
@@ -430,12 +461,12 @@ __global__ __launch_bounds__(64) void k_generic(GenArgs a) {
         // density (and Schwarz) blocks of the quartet into LDS: independent
         // loads issued together, read many times by the reductions below
         auto Pv = [&](int r, int c) { return P[(size_t)r * nb + c]; };
-        for (int i = lane; i < NAB; i += 64) {
+        for (int i = lane; i < NAB; i += NL) {
             const int af = pb.f0 + tb.ab_a[i], bf = pb.f1 + tb.ab_b[i];
             sP[0][i] = Pv(af, bf);
             if (!all_pass) sQ[0][i] = Qm[(size_t)af * nb + bf];
         }
-        for (int i = lane; i < NCD; i += 64) {
+        for (int i = lane; i < NCD; i += NL) {
             const int cf = pk.f0 + tb.cd_c[i], df = pk.f1 + tb.cd_d[i];
             sP[1][i] = Pv(cf, df);
             if (!all_pass) sQ[1][i] = Qm[(size_t)cf * nb + df];
@@ -444,16 +475,16 @@ __global__ __launch_bounds__(64) void k_generic(GenArgs a) {
         for (int sp = 0; sp < NK; ++sp) {
             const double *X = sp == 0 ? PKm : PK2m;
             double(*S)[100] = sP + 2 + 4 * sp;
-            for (int i = lane; i < NA * NC; i += 64) S[0][i] = X[(size_t)(pb.f0 + tb.ac_a[i]) * nb + pk.f0 + tb.ac_c[i]];
-            for (int i = lane; i < NA * ND; i += 64) S[1][i] = X[(size_t)(pb.f0 + tb.ad_a[i]) * nb + pk.f1 + tb.ad_d[i]];
-            for (int i = lane; i < NB * NC; i += 64) S[2][i] = X[(size_t)(pb.f1 + tb.bc_b[i]) * nb + pk.f0 + tb.bc_c[i]];
-            for (int i = lane; i < NB * ND; i += 64) S[3][i] = X[(size_t)(pb.f1 + tb.bd_b[i]) * nb + pk.f1 + tb.bd_d[i]];
+            for (int i = lane; i < NA * NC; i += NL) S[0][i] = X[(size_t)(pb.f0 + tb.ac_a[i]) * nb + pk.f0 + tb.ac_c[i]];
+            for (int i = lane; i < NA * ND; i += NL) S[1][i] = X[(size_t)(pb.f0 + tb.ad_a[i]) * nb + pk.f1 + tb.ad_d[i]];
+            for (int i = lane; i < NB * NC; i += NL) S[2][i] = X[(size_t)(pb.f1 + tb.bc_b[i]) * nb + pk.f0 + tb.bc_c[i]];
+            for (int i = lane; i < NB * ND; i += NL) S[3][i] = X[(size_t)(pb.f1 + tb.bd_b[i]) * nb + pk.f1 + tb.bd_d[i]];
         }
         __syncthreads();
         // apply degeneracy and the function-level Schwarz test in place
         {
             int ab = ab0, cd = cd0;
-            for (int i = lane; i < NABCD; i += 64) {
+            for (int i = lane; i < NABCD; i += NL) {
                 double m = deg;
                 if (!all_pass && sQ[0][ab] * sQ[1][cd] < a.tol) m = 0.0;
                 acc[i] *= m;
@@ -463,34 +494,86 @@ __global__ __launch_bounds__(64) void k_generic(GenArgs a) {
             }
         }
         __syncthreads();
-        auto at = [&](int ia, int ib, int ic, int id) { return acc[(ia * NB + ib) * NCD + ic * ND + id]; };
-        // A_ab += 2 sum_cd v P_cd ; A_cd += 2 sum_ab v P_ab
-        seg_reduce(lane, NAB, NCD, [&](int ab, int cd) { return acc[ab * NCD + cd] * sP[1][cd]; },
-                   [&](int ab, double v) {
-                       unsafeAtomicAdd(&Am[(size_t)(pb.f0 + tb.ab_a[ab]) * nb + pb.f1 + tb.ab_b[ab]], 2.0 * v);
-                   });
-        seg_reduce(lane, NCD, NAB, [&](int cd, int ab) { return acc[ab * NCD + cd] * sP[0][ab]; },
-                   [&](int cd, double v) {
-                       unsafeAtomicAdd(&Am[(size_t)(pk.f0 + tb.cd_c[cd]) * nb + pk.f1 + tb.cd_d[cd]], 2.0 * v);
-                   });
-        // B_ac += sum_bd v P_bd ; B_bc += sum_ad v P_ad ; B_ad += sum_bc v P_bc ; B_bd += sum_ac v P_ac
-        // (and the same into B2 with the second exchange density)
+        // compile-time contractions for the classes above 1080 integrals (and
+        // with several waves, which the shuffle reductions below cannot span);
+        // measured: (fd|dp) 13.2 -> 14.0 ms with them, (dd|dp) 5.8 -> 5.3 ms
+        if constexpr (TMPL && (NW > 1 || T_NABCD > 1080)) {
+            constexpr int cNA = ncart(TLA), cNB = ncart(TLB), cNC = ncart(TLC), cND = ncart(TLD);
+            constexpr int cNCD = cNC * cND, cNBCD = cNB * cNCD;
+            // A_ab += 2 sum_cd v P_cd ; A_cd += 2 sum_ab v P_ab
+            contract_ct<cNA * cNB, cNCD, NL>(
+                lane, acc, sP[1], [&](int o) { return o * cNCD; }, [](auto t_) { return (int)PK_CI(t_); },
+                [&](int o, double v) { unsafeAtomicAdd(&Am[(size_t)(pb.f0 + o / cNB) * nb + pb.f1 + o % cNB], 2.0 * v); });
+            contract_ct<cNCD, cNA * cNB, NL>(
+                lane, acc, sP[0], [&](int o) { return o; }, [](auto t_) { return (int)PK_CI(t_) * cNCD; },
+                [&](int o, double v) { unsafeAtomicAdd(&Am[(size_t)(pk.f0 + o / cND) * nb + pk.f1 + o % cND], 2.0 * v); });
 #pragma unroll
-        for (int sp = 0; sp < NK; ++sp) {
-            double *Bs = sp == 0 ? Bm : B2m;
-            const double(*S)[100] = sP + 2 + 4 * sp;  // ac, ad, bc, bd
-            seg_reduce(lane, NA * NC, NB * ND, [&](int o, int t) {
-                return at(tb.ac_a[o], tb.bd_b[t], tb.ac_c[o], tb.bd_d[t]) * S[3][t];
-            }, [&](int o, double v) { unsafeAtomicAdd(&Bs[(size_t)(pb.f0 + tb.ac_a[o]) * nb + pk.f0 + tb.ac_c[o]], v); });
-            seg_reduce(lane, NB * NC, NA * ND, [&](int o, int t) {
-                return at(tb.ad_a[t], tb.bc_b[o], tb.bc_c[o], tb.ad_d[t]) * S[1][t];
-            }, [&](int o, double v) { unsafeAtomicAdd(&Bs[(size_t)(pb.f1 + tb.bc_b[o]) * nb + pk.f0 + tb.bc_c[o]], v); });
-            seg_reduce(lane, NA * ND, NB * NC, [&](int o, int t) {
-                return at(tb.ad_a[o], tb.bc_b[t], tb.bc_c[t], tb.ad_d[o]) * S[2][t];
-            }, [&](int o, double v) { unsafeAtomicAdd(&Bs[(size_t)(pb.f0 + tb.ad_a[o]) * nb + pk.f1 + tb.ad_d[o]], v); });
-            seg_reduce(lane, NB * ND, NA * NC, [&](int o, int t) {
-                return at(tb.ac_a[t], tb.bd_b[o], tb.ac_c[t], tb.bd_d[o]) * S[0][t];
-            }, [&](int o, double v) { unsafeAtomicAdd(&Bs[(size_t)(pb.f1 + tb.bd_b[o]) * nb + pk.f1 + tb.bd_d[o]], v); });
+            for (int sp = 0; sp < NK; ++sp) {
+                double *Bs = sp == 0 ? Bm : B2m;
+                const double(*S)[100] = sP + 2 + 4 * sp;  // ac, ad, bc, bd
+                // B_ac += sum_bd v P_bd
+                contract_ct<cNA * cNC, cNB * cND, NL>(
+                    lane, acc, S[3], [&](int o) { return (o / cNC) * cNBCD + (o % cNC) * cND; },
+                    [](auto t_) {
+                        constexpr int t = PK_CI(t_);
+                        return (t / cND) * cNCD + t % cND;
+                    },
+                    [&](int o, double v) { unsafeAtomicAdd(&Bs[(size_t)(pb.f0 + o / cNC) * nb + pk.f0 + o % cNC], v); });
+                // B_bc += sum_ad v P_ad
+                contract_ct<cNB * cNC, cNA * cND, NL>(
+                    lane, acc, S[1], [&](int o) { return (o / cNC) * cNCD + (o % cNC) * cND; },
+                    [](auto t_) {
+                        constexpr int t = PK_CI(t_);
+                        return (t / cND) * cNBCD + t % cND;
+                    },
+                    [&](int o, double v) { unsafeAtomicAdd(&Bs[(size_t)(pb.f1 + o / cNC) * nb + pk.f0 + o % cNC], v); });
+                // B_ad += sum_bc v P_bc
+                contract_ct<cNA * cND, cNB * cNC, NL>(
+                    lane, acc, S[2], [&](int o) { return (o / cND) * cNBCD + o % cND; },
+                    [](auto t_) {
+                        constexpr int t = PK_CI(t_);
+                        return (t / cNC) * cNCD + (t % cNC) * cND;
+                    },
+                    [&](int o, double v) { unsafeAtomicAdd(&Bs[(size_t)(pb.f0 + o / cND) * nb + pk.f1 + o % cND], v); });
+                // B_bd += sum_ac v P_ac
+                contract_ct<cNB * cND, cNA * cNC, NL>(
+                    lane, acc, S[0], [&](int o) { return (o / cND) * cNCD + o % cND; },
+                    [](auto t_) {
+                        constexpr int t = PK_CI(t_);
+                        return (t / cNC) * cNBCD + (t % cNC) * cND;
+                    },
+                    [&](int o, double v) { unsafeAtomicAdd(&Bs[(size_t)(pb.f1 + o / cND) * nb + pk.f1 + o % cND], v); });
+            }
+        } else {
+            auto at = [&](int ia, int ib, int ic, int id) { return acc[(ia * NB + ib) * NCD + ic * ND + id]; };
+            // A_ab += 2 sum_cd v P_cd ; A_cd += 2 sum_ab v P_ab
+            seg_reduce(lane, NAB, NCD, [&](int ab, int cd) { return acc[ab * NCD + cd] * sP[1][cd]; },
+                       [&](int ab, double v) {
+                           unsafeAtomicAdd(&Am[(size_t)(pb.f0 + tb.ab_a[ab]) * nb + pb.f1 + tb.ab_b[ab]], 2.0 * v);
+                       });
+            seg_reduce(lane, NCD, NAB, [&](int cd, int ab) { return acc[ab * NCD + cd] * sP[0][ab]; },
+                       [&](int cd, double v) {
+                           unsafeAtomicAdd(&Am[(size_t)(pk.f0 + tb.cd_c[cd]) * nb + pk.f1 + tb.cd_d[cd]], 2.0 * v);
+                       });
+            // B_ac += sum_bd v P_bd ; B_bc += sum_ad v P_ad ; B_ad += sum_bc v P_bc ; B_bd += sum_ac v P_ac
+            // (and the same into B2 with the second exchange density)
+    #pragma unroll
+            for (int sp = 0; sp < NK; ++sp) {
+                double *Bs = sp == 0 ? Bm : B2m;
+                const double(*S)[100] = sP + 2 + 4 * sp;  // ac, ad, bc, bd
+                seg_reduce(lane, NA * NC, NB * ND, [&](int o, int t) {
+                    return at(tb.ac_a[o], tb.bd_b[t], tb.ac_c[o], tb.bd_d[t]) * S[3][t];
+                }, [&](int o, double v) { unsafeAtomicAdd(&Bs[(size_t)(pb.f0 + tb.ac_a[o]) * nb + pk.f0 + tb.ac_c[o]], v); });
+                seg_reduce(lane, NB * NC, NA * ND, [&](int o, int t) {
+                    return at(tb.ad_a[t], tb.bc_b[o], tb.bc_c[o], tb.ad_d[t]) * S[1][t];
+                }, [&](int o, double v) { unsafeAtomicAdd(&Bs[(size_t)(pb.f1 + tb.bc_b[o]) * nb + pk.f0 + tb.bc_c[o]], v); });
+                seg_reduce(lane, NA * ND, NB * NC, [&](int o, int t) {
+                    return at(tb.ad_a[o], tb.bc_b[t], tb.bc_c[t], tb.ad_d[o]) * S[2][t];
+                }, [&](int o, double v) { unsafeAtomicAdd(&Bs[(size_t)(pb.f0 + tb.ad_a[o]) * nb + pk.f1 + tb.ad_d[o]], v); });
+                seg_reduce(lane, NB * ND, NA * NC, [&](int o, int t) {
+                    return at(tb.ac_a[t], tb.bd_b[o], tb.ac_c[t], tb.bd_d[o]) * S[0][t];
+                }, [&](int o, double v) { unsafeAtomicAdd(&Bs[(size_t)(pb.f1 + tb.bd_b[o]) * nb + pk.f1 + tb.bd_d[o]], v); });
+            }
         }
         __syncthreads();
     }

@@ -325,16 +325,20 @@ void free_work(pk_ctx *c) {
 // One 64-lane workgroup per item; the dynamic LDS holds the transferred 1-D
 // factors and the quartet's accumulators (at most 123 KB for (ff|ff)).
 // Cooperative kernels with the class fixed at compile time, for the J/K lists
-// of the classes without a per-lane kernel (null elsewhere).
+// of the classes without a per-lane kernel (null elsewhere), with
+// pk::gen_waves(class) waves per quartet.
 struct GenKernels {
     const void *k1, *k2;  // one and two exchange densities
+    int waves;
 };
 template <int A, int B, int C, int D>
 constexpr GenKernels gen_kernels() {
-    if constexpr (!pk::rys_tmpl_available(A, B, C, D))
-        return GenKernels{(const void *)pk::k_generic<1, A, B, C, D>, (const void *)pk::k_generic<2, A, B, C, D>};
-    else
-        return GenKernels{nullptr, nullptr};
+    if constexpr (!pk::rys_tmpl_available(A, B, C, D)) {
+        constexpr int W = pk::gen_waves(A, B, C, D);
+        return GenKernels{(const void *)pk::k_generic<1, A, B, C, D, W>, (const void *)pk::k_generic<2, A, B, C, D, W>, W};
+    } else {
+        return GenKernels{nullptr, nullptr, 1};
+    }
 }
 #define PK_GEN_ENTRY(a, b, c, d, shard) gen_kernels<a, b, c, d>(),
 const GenKernels kGenKernels[kQuartClasses] = {PK_FOR_EACH_CLASS(PK_GEN_ENTRY)};
@@ -356,7 +360,7 @@ int launch_generic(const pk::GenArgs &a, hipStream_t s) {
             }
             pk::GenArgs x = a;
             void *args[] = {&x};
-            PK_HIP(hipLaunchKernel(k, dim3(a.nitems), dim3(64), args, bytes, s));
+            PK_HIP(hipLaunchKernel(k, dim3(a.nitems), dim3(64 * g.waves), args, bytes, s));
             return PK_OK;
         }
     }
