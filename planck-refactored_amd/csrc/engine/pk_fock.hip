@@ -204,6 +204,7 @@ struct ClassWork {
     int kernels = 0;
     double ms_total = 0.0;
     int64_t builds = 0;
+    double cost_ms = 0.0;  // measured time of the list in the last timed build (0: not yet)
     hipEvent_t e0 = nullptr, e1 = nullptr;
 };
 
@@ -280,6 +281,8 @@ struct pk_ctx {
     hipEvent_t fork_ev = nullptr, join_ev[kSideQueues] = {};
     bool side_ok = false;
     bool side_off = false;  // PK_SIDE_QUEUES=0
+    bool cost_learned = false;  // the lists' times have been measured (first large build)
+    bool learning = false;      // the current build records per-list events for that
     bool profile = false;
     pk_stats st{};
 };
@@ -317,6 +320,7 @@ void free_work(pk_ctx *c) {
     }
     c->work.clear();
     c->work_ok = false;
+    c->cost_learned = false;
     drop_graph(c);
 }
 
@@ -697,14 +701,23 @@ int launch_classes(pk_ctx *c, const BuildBufs &sb, bool events, int &nl) {
         }
         std::vector<size_t> order(c->work.size());
         for (size_t i = 0; i < order.size(); ++i) order[i] = i;
-        std::stable_sort(order.begin(), order.end(),
-                         [&](size_t x, size_t y) { return c->work[x].flops > c->work[y].flops; });
+        // cost of a list: its measured time when the context has one (the first
+        // large build of a context times every list, one at a time), else the
+        // model flops over a typical rate of its kernel (TFLOP/s: per-lane OS
+        // ~10, per-lane Rys ~3, cooperative ~1.5)
+        std::vector<double> cost(c->work.size());
+        for (size_t i = 0; i < cost.size(); ++i) {
+            const ClassWork &w = c->work[i];
+            const int kind = list_kind(c, w) % 3;  // 0 OS, 1 Rys, 2 cooperative
+            cost[i] = w.cost_ms > 0.0 ? w.cost_ms : w.flops * 1e-9 / (kind == 0 ? 10.0 : kind == 1 ? 3.0 : 1.5);
+        }
+        std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return cost[x] > cost[y]; });
         double load[kSideQueues + 1] = {};
         for (size_t i : order) {
             int k = 0;
             for (int j = 1; j <= kSideQueues; ++j)
                 if (load[j] < load[k]) k = j;
-            load[k] += c->work[i].flops;
+            load[k] += cost[i];
             if ((rc = launch_list(c, c->work[i], sb, false, q[k], nl)) != PK_OK) return rc;
         }
         for (int i = 0; i < kSideQueues; ++i) {
@@ -882,9 +895,13 @@ int build_device(pk_ctx *c, const double *dP, double *dJ, double *dK, double *dG
     if ((rc = ensure_buffers(c)) != PK_OK) return rc;
     hipStream_t s = c->stream;
     const size_t bytes = c->n2 * sizeof(double);
-    if (!c->graph_on || c->profile) {
+    // the first build that would use the side queues times every list (one
+    // at a time), so that later builds balance the queues by measured cost
+    c->learning = !c->cost_learned && !c->profile && c->side_ok && !c->side_off &&
+                  c->st.quartets >= kSideMinQuartets && c->work.size() > 1;
+    if (!c->graph_on || c->profile || c->learning) {
         PK_HIP(hipEventRecord(c->ev0, s));
-        if ((rc = launch_build(c, dP, dJ, dK, dG, ctx_bufs(c), c->profile)) != PK_OK) return rc;
+        if ((rc = launch_build(c, dP, dJ, dK, dG, ctx_bufs(c), c->profile || c->learning)) != PK_OK) return rc;
         PK_HIP(hipEventRecord(c->ev1, s));
         return PK_OK;
     }
@@ -919,16 +936,21 @@ int finish_timing(pk_ctx *c) {
     PK_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->st.last_build_ms = ms;
     c->st.last_eri_kernel_ms = ms;
-    if (c->profile) {
+    if (c->profile || c->learning) {
         double eri = 0.0;
         for (ClassWork &w : c->work) {
             float m = 0.f;
             PK_HIP(hipEventElapsedTime(&m, w.e0, w.e1));
-            w.ms_total += m;
-            w.builds += 1;
+            if (c->profile) {
+                w.ms_total += m;
+                w.builds += 1;
+            }
+            w.cost_ms = m;
             eri += m;
         }
         c->st.last_eri_kernel_ms = eri;
+        c->cost_learned = true;
+        c->learning = false;
     }
     return PK_OK;
 }
