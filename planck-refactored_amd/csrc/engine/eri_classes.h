@@ -141,4 +141,15 @@ void launch_jk_os(const JKArgs &a, int grid, int block, hipStream_t s, int &nlau
     }
 }
 
+// the segment-mode variant of a seg_alt_class (no launch elsewhere)
+template <typename R, int LA, int LB, int LC, int LD, int NK = 1>
+void launch_jk_os_seg(const JKArgs &a, int grid, int block, hipStream_t s, int &nlaunch) {
+    if constexpr (os_available(LA, LB, LC, LD) && seg_alt_class(LA, LB, LC, LD)) {
+        constexpr int NCD = ncart(LC) * ncart(LD);
+        hipLaunchKernelGGL((k_jk<R, LA, LB, LC, LD, ALG_OS, 0, NCD, NK, 1>), dim3(grid), dim3(block),
+                           jk_lds_bytes(a, LA, LB, NK), s, a);
+        ++nlaunch;
+    }
+}
+
 }  // namespace pk

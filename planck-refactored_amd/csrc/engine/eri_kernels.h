@@ -137,6 +137,16 @@ constexpr bool seg_class(int la, int lb, int lc, int ld) {
     return la == 2 && lb == 0 && lc == 2;
 }
 
+// Classes whose plain kernel is also compiled in segment mode, chosen at run
+// time for lists of few primitive quartets per shell quartet: (ss|ss), (ps|ss),
+// (ps|ps), (pp|ss).  In 6-31G* water clusters (about 11-13 primitive quartets
+// per shell quartet) they do ~5 exchange atomics per quartet and are bound by
+// atomic throughput; in cc-pVTZ alkanes (40-80) by the primitive loop, where
+// the plain kernel is faster.
+constexpr bool seg_alt_class(int la, int lb, int lc, int ld) {
+    return !seg_class(la, lb, lc, ld) && la + lb + lc + ld <= 2;
+}
+
 // J/K kernel: one wave = one bra pair and up to 64 ket pairs.  The integrals
 // are computed in R; the J and K accumulators are FP64.
 // NK = 2: one-pass UHF digestion, J of P (= Pa + Pb) into A and the exchange of
@@ -155,13 +165,14 @@ constexpr bool seg_class(int la, int lb, int lc, int ld) {
 // later load of the wave waits for them); this replaces ~NA ND + NB ND of them
 // per quartet by a share of one flush per segment.  J_cd stays a global atomic
 // per ket component.
-template <typename R, int LA, int LB, int LC, int LD, int ALG, int C0, int C1, int NK = 1>
+// SEGV: 1 / 0 compiles the segment-mode / plain kernel, -1 takes seg_class.
+template <typename R, int LA, int LB, int LC, int LD, int ALG, int C0, int C1, int NK = 1, int SEGV = -1>
 __global__ __launch_bounds__(256) void k_jk(JKArgs a) {
     using K = Cls<LA, LB, LC, LD>;
     constexpr int NA = K::NA, NB = K::NB, ND = K::ND, NAB = K::NAB, NE = K::NE, NCH = C1 - C0;
     constexpr int NROW = NA + NB;
     extern __shared__ double krow[];
-    constexpr bool SEG = seg_class(LA, LB, LC, LD);
+    constexpr bool SEG = SEGV >= 0 ? SEGV != 0 : seg_class(LA, LB, LC, LD);
     const bool segmode = SEG && a.segmode != 0;
     int w_first, w_end, w_step, w_lead;
     if (segmode) {

@@ -35,7 +35,10 @@
     void pk_launch_jk_os_##a##b##c##d(const pk::JKArgs &, int, int, hipStream_t, int &);            \
     void pk_launch_jk_rys32_##a##b##c##d(const pk::JKArgs &, int, int, hipStream_t, int &);         \
     void pk_launch_jk_os32_##a##b##c##d(const pk::JKArgs &, int, int, hipStream_t, int &);         \
-    void pk_launch_jk_os_uhf_##a##b##c##d(const pk::JKArgs &, int, int, hipStream_t, int &);
+    void pk_launch_jk_os_uhf_##a##b##c##d(const pk::JKArgs &, int, int, hipStream_t, int &);          \
+    void pk_launch_jk_os_seg_##a##b##c##d(const pk::JKArgs &, int, int, hipStream_t, int &);          \
+    void pk_launch_jk_os32_seg_##a##b##c##d(const pk::JKArgs &, int, int, hipStream_t, int &);        \
+    void pk_launch_jk_os_uhf_seg_##a##b##c##d(const pk::JKArgs &, int, int, hipStream_t, int &);
 PK_FOR_EACH_CLASS(PK_DECL)
 #undef PK_DECL
 
@@ -65,6 +68,9 @@ constexpr int64_t kSideMinQuartets = 1 << 20;
 // Segment mode of the per-lane kernels (see k_jk): at most kSegWaves waves of
 // one bra per segment, and at most kSegLdsMax bytes of LDS exchange rows.
 constexpr int kSegWaves = 16;
+// primitive quartets per shell quartet below which a list of a pk::seg_alt_class
+// runs in segment mode (6-31G* water clusters: 11-13; cc-pVTZ alkanes: 40-80)
+constexpr double kSegAltPrim = 24.0;
 constexpr size_t kSegLdsMax = 64 * 1024;
 constexpr int kPairClasses = (kMaxL + 1) * (kMaxL + 2) / 2;
 constexpr int kQuartClasses = kPairClasses * (kPairClasses + 1) / 2;
@@ -74,19 +80,24 @@ struct ClassFns {
     bool has_os;    // Obara-Saika per-lane kernel
     bool has_tmpl;  // Rys per-lane kernel (else the cooperative kernel)
     bool seg;       // the per-lane kernels run in segment mode (pk::seg_class)
+    bool seg_alt;   // the OS kernel also exists in segment mode (pk::seg_alt_class)
     void (*jk_os)(const pk::JKArgs &, int, int, hipStream_t, int &);
     void (*jk_rys)(const pk::JKArgs &, int, int, hipStream_t, int &);
     void (*jk_os32)(const pk::JKArgs &, int, int, hipStream_t, int &);  // FP32 integrals
     void (*jk_rys32)(const pk::JKArgs &, int, int, hipStream_t, int &);
     void (*jk_os_uhf)(const pk::JKArgs &, int, int, hipStream_t, int &);  // FP64, two exchange densities
+    void (*jk_os_seg)(const pk::JKArgs &, int, int, hipStream_t, int &);  // segment-mode variants
+    void (*jk_os32_seg)(const pk::JKArgs &, int, int, hipStream_t, int &);
+    void (*jk_os_uhf_seg)(const pk::JKArgs &, int, int, hipStream_t, int &);
 };
 
 #define PK_ENTRY(a, b, c, d, shard)                                                             \
     ClassFns{a, b, c, d, pk::os_available(a, b, c, d), pk::rys_tmpl_available(a, b, c, d),      \
-             pk::seg_class(a, b, c, d),                                                         \
+             pk::seg_class(a, b, c, d), pk::os_available(a, b, c, d) && pk::seg_alt_class(a, b, c, d), \
              &pk_launch_jk_os_##a##b##c##d, &pk_launch_jk_rys_##a##b##c##d,                     \
              &pk_launch_jk_os32_##a##b##c##d, &pk_launch_jk_rys32_##a##b##c##d,                 \
-             &pk_launch_jk_os_uhf_##a##b##c##d},
+             &pk_launch_jk_os_uhf_##a##b##c##d, &pk_launch_jk_os_seg_##a##b##c##d,                   \
+             &pk_launch_jk_os32_seg_##a##b##c##d, &pk_launch_jk_os_uhf_seg_##a##b##c##d},
 // Quartet classes (bra class >= ket class), indexed cb (cb + 1) / 2 + ck with
 // pair classes (0,0) (1,0) (1,1) (2,0) (2,1) (2,2) (3,0) (3,1) (3,2) (3,3).
 const ClassFns kClasses[kQuartClasses] = {PK_FOR_EACH_CLASS(PK_ENTRY)};
@@ -773,17 +784,21 @@ int launch_list(pk_ctx *c, ClassWork &w, const BuildBufs &sb, bool events, hipSt
         const bool fp32 = w.prec == PK_PREC_MIXED && !w.alt;
         const int wpb = w.block / 64;
         // segment mode when the exchange rows of the call fit the LDS budget
-        // (two rows sets for the one-pass UHF OS kernels)
+        // (two rows sets for the one-pass UHF OS kernels); the low-L OS classes
+        // compiled both ways take it for lists of few primitive quartets per
+        // shell quartet (atomic-bound), the plain kernel otherwise
+        const bool seg_alt = f.seg_alt && use_os(c, f) && (double)w.nprimq <= kSegAltPrim * (double)w.nquart;
         {
             const int nk = uhf && use_os(c, f) ? 2 : 1;
             const size_t rows = (size_t)nk * (pk::ncart(f.la) + pk::ncart(f.lb)) * w.kcols +
                                 (size_t)pk::ncart(f.la) * pk::ncart(f.lb);
-            a.segmode = c->seg_on && f.seg && w.nseg > 0 && rows * sizeof(double) <= kSegLdsMax;
+            a.segmode = c->seg_on && (f.seg || seg_alt) && w.nseg > 0 && rows * sizeof(double) <= kSegLdsMax;
         }
+        const bool alt = seg_alt && a.segmode;  // the segment-mode variant of a plain OS class
         int grid = a.segmode ? w.nseg : (w.nwaves + wpb - 1) / wpb;
         if (w.remap) grid = (grid + 7) / 8 * 8;
         if (uhf && use_os(c, f)) {
-            f.jk_os_uhf(a, grid, w.block, s, nl);
+            (alt ? f.jk_os_uhf_seg : f.jk_os_uhf)(a, grid, w.block, s, nl);
         } else if (uhf && f.has_tmpl) {
             // per-lane Rys kernels once per spin density: J of each into A
             // (summing to J of Pa + Pb), exchange into B and B2
@@ -796,7 +811,10 @@ int launch_list(pk_ctx *c, ClassWork &w, const BuildBufs &sb, bool events, hipSt
         } else if (uhf) {
             // handled by the cooperative kernel below
         } else if (use_os(c, f)) {
-            (fp32 ? f.jk_os32 : f.jk_os)(a, grid, w.block, s, nl);
+            if (alt)
+                (fp32 ? f.jk_os32_seg : f.jk_os_seg)(a, grid, w.block, s, nl);
+            else
+                (fp32 ? f.jk_os32 : f.jk_os)(a, grid, w.block, s, nl);
         } else if (f.has_tmpl && !(w.alt && w.prec == PK_PREC_FP64)) {
             (fp32 ? f.jk_rys32 : f.jk_rys)(a, grid, w.block, s, nl);
         }

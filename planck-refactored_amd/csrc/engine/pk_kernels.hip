@@ -20,6 +20,15 @@
     }                                                                                          \
     void pk_launch_jk_os_uhf_##a##b##c##d(const pk::JKArgs &x, int g, int b_, hipStream_t s, int &n) { \
         pk::launch_jk_os<double, a, b, c, d, 2>(x, g, b_, s, n);                                   \
+    }                                                                                          \
+    void pk_launch_jk_os_seg_##a##b##c##d(const pk::JKArgs &x, int g, int b_, hipStream_t s, int &n) { \
+        pk::launch_jk_os_seg<double, a, b, c, d>(x, g, b_, s, n);                                  \
+    }                                                                                          \
+    void pk_launch_jk_os32_seg_##a##b##c##d(const pk::JKArgs &x, int g, int b_, hipStream_t s, int &n) { \
+        pk::launch_jk_os_seg<float, a, b, c, d>(x, g, b_, s, n);                                   \
+    }                                                                                          \
+    void pk_launch_jk_os_uhf_seg_##a##b##c##d(const pk::JKArgs &x, int g, int b_, hipStream_t s, int &n) { \
+        pk::launch_jk_os_seg<double, a, b, c, d, 2>(x, g, b_, s, n);                               \
     }
 
 #define PK_SKIP(a, b, c, d)
