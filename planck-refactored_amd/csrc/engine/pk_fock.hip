@@ -292,6 +292,7 @@ struct pk_ctx {
     hipEvent_t fork_ev = nullptr, join_ev[kSideQueues] = {};
     bool side_ok = false;
     bool side_off = false;  // PK_SIDE_QUEUES=0
+    double seg_alt_prim = kSegAltPrim;  // PK_SEG_ALT_PRIM
     bool cost_learned = false;  // the lists' times have been measured (first large build)
     bool learning = false;      // the current build records per-list events for that
     bool profile = false;
@@ -787,7 +788,7 @@ int launch_list(pk_ctx *c, ClassWork &w, const BuildBufs &sb, bool events, hipSt
         // (two rows sets for the one-pass UHF OS kernels); the low-L OS classes
         // compiled both ways take it for lists of few primitive quartets per
         // shell quartet (atomic-bound), the plain kernel otherwise
-        const bool seg_alt = f.seg_alt && use_os(c, f) && (double)w.nprimq <= kSegAltPrim * (double)w.nquart;
+        const bool seg_alt = f.seg_alt && use_os(c, f) && (double)w.nprimq <= c->seg_alt_prim * (double)w.nquart;
         {
             const int nk = uhf && use_os(c, f) ? 2 : 1;
             const size_t rows = (size_t)nk * (pk::ncart(f.la) + pk::ncart(f.lb)) * w.kcols +
@@ -1185,6 +1186,9 @@ int ctx_build(const pk_basis_desc *descs, int nmol, pk_ctx **out) {
     // diagnostic switch: PK_SIDE_QUEUES=0 keeps every class kernel of a build
     // on the context stream (one kernel at a time, as a kernel trace times them)
     if (const char *e = std::getenv("PK_SIDE_QUEUES")) c->side_off = std::atoi(e) == 0;
+    // diagnostic knob: PK_SEG_ALT_PRIM overrides the primitive quartets per
+    // shell quartet below which the low-L OS lists run in segment mode
+    if (const char *e = std::getenv("PK_SEG_ALT_PRIM")) c->seg_alt_prim = std::atof(e);
     c->st.shell_pairs = (int64_t)c->pairs.size();
     int rc = PK_OK;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
