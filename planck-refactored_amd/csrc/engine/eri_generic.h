@@ -121,15 +121,60 @@ static __device__ const unsigned char kCartTab[4][10][3] = {
     {{3, 0, 0}, {2, 1, 0}, {2, 0, 1}, {1, 2, 0}, {1, 1, 1}, {1, 0, 2}, {0, 3, 0}, {0, 2, 1}, {0, 1, 2}, {0, 0, 3}}};
 
 // Index tables of one class, built once per workgroup so that the hot loops
-// decode their indices by lookups instead of integer divisions.
+// decode their indices by lookups instead of integer divisions.  NPR bounds
+// the component pairs of every shell pair (ab, cd, ac, ad, bc, bd), NGT the
+// transferred 1-D factors per (root, axis): the largest class's 100 and 256,
+// or the sizes of the class a kernel is compiled for (less static LDS, more
+// workgroups per CU).
+template <int NPR, int NGT>
 struct GenTabs {
-    unsigned char ab_a[100], ab_b[100], cd_c[100], cd_d[100];  // ab -> (a, b), cd -> (c, d)
-    unsigned char ac_a[100], ac_c[100], bd_b[100], bd_d[100];  // pairs of the exchange sums
-    unsigned char bc_b[100], bc_c[100], ad_a[100], ad_d[100];
-    unsigned short ab_g[100][3];                               // G offset of (a, b) per axis
-    unsigned char cd_g[100][3];                                // G offset of (c, d) per axis
-    unsigned char g_a[256], g_b[256], g_c[256], g_d[256];      // G index -> exponents
+    unsigned char ab_a[NPR], ab_b[NPR], cd_c[NPR], cd_d[NPR];  // ab -> (a, b), cd -> (c, d)
+    unsigned char ac_a[NPR], ac_c[NPR], bd_b[NPR], bd_d[NPR];  // pairs of the exchange sums
+    unsigned char bc_b[NPR], bc_c[NPR], ad_a[NPR], ad_d[NPR];
+    unsigned short ab_g[NPR][3];                               // G offset of (a, b) per axis
+    unsigned char cd_g[NPR][3];                                // G offset of (c, d) per axis
+    unsigned char g_a[NGT], g_b[NGT], g_c[NGT], g_d[NGT];      // G index -> exponents
 };
+
+// Component pairs bound of a class: the largest of NA NB, NC ND, NA NC,
+// NA ND, NB NC, NB ND.
+constexpr int gen_pair_bound(int la, int lb, int lc, int ld) {
+    const int na = ncart(la), nb = ncart(lb), nc = ncart(lc), nd = ncart(ld);
+    int m = na * nb;
+    m = m > nc * nd ? m : nc * nd;
+    m = m > na * nc ? m : na * nc;
+    m = m > na * nd ? m : na * nd;
+    m = m > nb * nc ? m : nb * nc;
+    return m > nb * nd ? m : nb * nd;
+}
+
+// Register accumulation (class fixed at compile time, at most GEN_RACC_MAX
+// components per lane of the NL lanes of a quartet, GEN_RACC1_MAX with one
+// wave per quartet).  The quartet's integrals then reach LDS only for the
+// digestion, over the (by then dead) 2-D and transferred factors when those
+// are at least as large.
+#ifndef PK_GEN_RACC_MAX
+#define PK_GEN_RACC_MAX 15
+#endif
+#ifndef PK_GEN_RACC1_MAX
+#define PK_GEN_RACC1_MAX 17
+#endif
+constexpr int GEN_RACC_MAX = PK_GEN_RACC_MAX, GEN_RACC1_MAX = PK_GEN_RACC1_MAX;
+constexpr int gen_regs_per_lane(int la, int lb, int lc, int ld, int nl) {
+    return (ncart(la) * ncart(lb) * ncart(lc) * ncart(ld) + nl - 1) / nl;
+}
+constexpr bool gen_racc(int la, int lb, int lc, int ld, int nl) {
+    return gen_regs_per_lane(la, lb, lc, ld, nl) <= (nl == 64 ? GEN_RACC1_MAX : GEN_RACC_MAX);
+}
+// Waves per SIMD the compiler must leave room for: two (256 registers) for the
+// one-wave kernels that accumulate more than GEN_RACC_MAX components per lane
+// in registers (without the bound they take 256 VGPRs and a few AGPRs, one
+// wave per SIMD); one elsewhere.
+constexpr int gen_min_waves(int la, int lb, int lc, int ld, int nl) {
+    return la >= 0 && nl == 64 && gen_racc(la, lb, lc, ld, nl) && gen_regs_per_lane(la, lb, lc, ld, nl) > GEN_RACC_MAX
+               ? 2
+               : 1;
+}
 
 // NK = 2: one-pass UHF digestion (J of P into A, exchange of PK into B and of
 // PK2 into B2).  TLA..TLD >= 0 fix the class at compile time (every size,
@@ -139,25 +184,31 @@ struct GenTabs {
 // least 1800 integrals per quartet (measured on n-C10H22 / cc-pVTZ: (dd|dp)
 // 5.3 -> 3.5 ms, (fd|fd) 3.2 -> 1.9 ms, (ff|dd) 2.2 -> 1.3 ms), one below
 // ((fd|dp) 14.0 -> 15.0 ms, (fp|fp) 3.8 -> 4.6 ms with two).
+#ifndef PK_GEN_NW2_MIN
+#define PK_GEN_NW2_MIN 1800
+#endif
 constexpr int gen_waves(int la, int lb, int lc, int ld) {
-    return ncart(la) * ncart(lb) * ncart(lc) * ncart(ld) >= 1800 ? 2 : 1;
+    return ncart(la) * ncart(lb) * ncart(lc) * ncart(ld) >= PK_GEN_NW2_MIN ? 2 : 1;
 }
 
 // NW waves per workgroup (NL = 64 NW lanes) share each quartet; NW > 1 needs
 // the class at compile time.
 template <int NK, int TLA = -1, int TLB = -1, int TLC = -1, int TLD = -1, int NW = 1>
-__global__ __launch_bounds__(64 * NW) void k_generic(GenArgs a) {
+__global__ __launch_bounds__(64 * NW, gen_min_waves(TLA, TLB, TLC, TLD, 64 * NW)) void k_generic(GenArgs a) {
     constexpr int NL = 64 * NW;
     static_assert(NW == 1 || (TLA >= 0 && TLB >= 0 && TLC >= 0 && TLD >= 0),
                   "several waves per quartet need the class at compile time (shuffle-free digestion)");
+    constexpr bool TMPL = TLA >= 0 && TLB >= 0 && TLC >= 0 && TLD >= 0;
+    constexpr int NPR = TMPL ? gen_pair_bound(TLA, TLB, TLC, TLD) : 100;
+    constexpr int NGT = TMPL ? (TLA + 1) * (TLB + 1) * (TLC + 1) * (TLD + 1) : 256;
     extern __shared__ double smem[];
     __shared__ double su[GEN_NR], sw[GEN_NR];
     __shared__ double hab[3][10], hcd[3][10];  // C(b,k) AB^(b-k) per axis at [b(b+1)/2 + k]; ket likewise
     // density blocks of the quartet: ab, cd (J density), ac, ad, bc, bd (exchange
     // density of B), then ac, ad, bc, bd of the exchange density of B2
-    __shared__ double sP[NK == 2 ? 10 : 6][100];
-    __shared__ double sQ[2][100];              // Schwarz blocks ab, cd
-    __shared__ GenTabs tb;
+    __shared__ double sP[NK == 2 ? 10 : 6][NPR];
+    __shared__ double sQ[2][NPR];              // Schwarz blocks ab, cd
+    __shared__ GenTabs<NPR, NGT> tb;
     const int lane = threadIdx.x;
     const int item = blockIdx.x;
     if (item >= a.nitems) return;
@@ -170,9 +221,27 @@ __global__ __launch_bounds__(64 * NW) void k_generic(GenArgs a) {
     const int F1 = F + 1, NI = (E + 1) * F1;
     const int D2 = lb + 1, D3 = lc + 1, D4 = ld + 1, D34 = D3 * D4;
     const int NG = (la + 1) * D2 * D34;  // transferred 1-D factors per (root, axis)
+    // Class fixed at compile time and at most GEN_RACC_MAX (one wave:
+    // GEN_RACC1_MAX) components per lane (measured on n-C10H22 / cc-pVTZ:
+    // (31|21) 14.2 -> 11.1 ms, (31|31) 3.8 -> 3.0 ms; with 17 components per
+    // lane, held to two waves per SIMD and the integrals over the dead factors
+    // in LDS, (32|21) 11.2 -> 8.0 ms and (31|22) 5.3 -> 3.9 ms, from LDS-bound
+    // occupancy of 6 to 8 workgroups per CU):
+    // the lane's components i = lane + 64 k are accumulated in registers, with
+    // their three G offsets (each < 256) packed in one word per component,
+    // computed once; the lane's transfer entries t = lane + 64 j likewise carry
+    // (root-axis, G index, exponents) packed in one word each.
+    constexpr int T_NABCD = TMPL ? ncart(TLA) * ncart(TLB) * ncart(TLC) * ncart(TLD) : 1;
+    constexpr int T_N = TMPL ? (TLA + TLB + TLC + TLD) / 2 + 1 : 1;
+    constexpr int T_NG = TMPL ? (TLA + 1) * (TLB + 1) * (TLC + 1) * (TLD + 1) : 1;
+    constexpr int KREG = (T_NABCD + NL - 1) / NL, KTR = (3 * T_N * T_NG + NL - 1) / NL;
+    constexpr bool RACC = TMPL && gen_racc(TLA, TLB, TLC, TLD, NL);
+    // register accumulation: the integrals take the place of the 2-D and
+    // transferred factors for the digestion when they fit there
+    constexpr bool ALIAS = RACC && T_NABCD <= 3 * T_N * ((TLA + TLB + 1) * (TLC + TLD + 1) + T_NG);
     double *I2 = smem;                   // [n * 3][E + 1][F + 1]
     double *G = I2 + n * 3 * NI;         // [n * 3][la + 1][lb + 1][lc + 1][ld + 1]
-    double *acc = G + n * 3 * NG;        // [NABCD]
+    double *acc = ALIAS ? smem : G + n * 3 * NG;  // [NABCD]
 
     for (int i = lane; i < NAB; i += NL) {
         const int ia = i / NB, ib = i % NB;
@@ -206,21 +275,6 @@ __global__ __launch_bounds__(64 * NW) void k_generic(GenArgs a) {
     const int ra0 = lane / NG, g0 = lane % NG, sra = NL / NG, sg = NL % NG;
     __syncthreads();
 
-    // Class fixed at compile time and at most RACC_MAX components per lane
-    // (measured: (31|21) 14.2 -> 11.1 ms and (31|31) 3.8 -> 3.0 ms on
-    // n-C10H22 / cc-pVTZ at 2 waves per SIMD; the classes with 17 or more
-    // components per lane drop to 1 wave per SIMD and lose up to 19%):
-    // the lane's components i = lane + 64 k are accumulated in registers, with
-    // their three G offsets (each < 256) packed in one word per component,
-    // computed once; the lane's transfer entries t = lane + 64 j likewise carry
-    // (root-axis, G index, exponents) packed in one word each.
-    constexpr bool TMPL = TLA >= 0 && TLB >= 0 && TLC >= 0 && TLD >= 0;
-    constexpr int T_NABCD = TMPL ? ncart(TLA) * ncart(TLB) * ncart(TLC) * ncart(TLD) : 1;
-    constexpr int T_N = TMPL ? (TLA + TLB + TLC + TLD) / 2 + 1 : 1;
-    constexpr int T_NG = TMPL ? (TLA + 1) * (TLB + 1) * (TLC + 1) * (TLD + 1) : 1;
-    constexpr int KREG = (T_NABCD + NL - 1) / NL, KTR = (3 * T_N * T_NG + NL - 1) / NL;
-    constexpr int RACC_MAX = 15;
-    constexpr bool RACC = TMPL && KREG <= RACC_MAX;
     unsigned goff[RACC ? KREG : 1], gtr[RACC ? KTR : 1];
     double racc[RACC ? KREG : 1];
     if constexpr (RACC) {
@@ -474,7 +528,7 @@ __global__ __launch_bounds__(64 * NW) void k_generic(GenArgs a) {
 #pragma unroll
         for (int sp = 0; sp < NK; ++sp) {
             const double *X = sp == 0 ? PKm : PK2m;
-            double(*S)[100] = sP + 2 + 4 * sp;
+            double(*S)[NPR] = sP + 2 + 4 * sp;
             for (int i = lane; i < NA * NC; i += NL) S[0][i] = X[(size_t)(pb.f0 + tb.ac_a[i]) * nb + pk.f0 + tb.ac_c[i]];
             for (int i = lane; i < NA * ND; i += NL) S[1][i] = X[(size_t)(pb.f0 + tb.ad_a[i]) * nb + pk.f1 + tb.ad_d[i]];
             for (int i = lane; i < NB * NC; i += NL) S[2][i] = X[(size_t)(pb.f1 + tb.bc_b[i]) * nb + pk.f0 + tb.bc_c[i]];
@@ -510,7 +564,7 @@ __global__ __launch_bounds__(64 * NW) void k_generic(GenArgs a) {
 #pragma unroll
             for (int sp = 0; sp < NK; ++sp) {
                 double *Bs = sp == 0 ? Bm : B2m;
-                const double(*S)[100] = sP + 2 + 4 * sp;  // ac, ad, bc, bd
+                const double(*S)[NPR] = sP + 2 + 4 * sp;  // ac, ad, bc, bd
                 // B_ac += sum_bd v P_bd
                 contract_ct<cNA * cNC, cNB * cND, NL>(
                     lane, acc, S[3], [&](int o) { return (o / cNC) * cNBCD + (o % cNC) * cND; },
@@ -560,7 +614,7 @@ __global__ __launch_bounds__(64 * NW) void k_generic(GenArgs a) {
     #pragma unroll
             for (int sp = 0; sp < NK; ++sp) {
                 double *Bs = sp == 0 ? Bm : B2m;
-                const double(*S)[100] = sP + 2 + 4 * sp;  // ac, ad, bc, bd
+                const double(*S)[NPR] = sP + 2 + 4 * sp;  // ac, ad, bc, bd
                 seg_reduce(lane, NA * NC, NB * ND, [&](int o, int t) {
                     return at(tb.ac_a[o], tb.bd_b[t], tb.ac_c[o], tb.bd_d[t]) * S[3][t];
                 }, [&](int o, double v) { unsafeAtomicAdd(&Bs[(size_t)(pb.f0 + tb.ac_a[o]) * nb + pk.f0 + tb.ac_c[o]], v); });
@@ -580,13 +634,16 @@ __global__ __launch_bounds__(64 * NW) void k_generic(GenArgs a) {
 }
 
 // dynamic LDS of the generic kernel for a class: 2-D integrals and transferred
-// factors for n roots x 3 axes, and the quartet's accumulators
-inline size_t generic_lds_bytes(int la, int lb, int lc, int ld) {
-    const size_t nabcd = (size_t)((la + 1) * (la + 2) / 2) * ((lb + 1) * (lb + 2) / 2) * ((lc + 1) * (lc + 2) / 2) *
-                         ((ld + 1) * (ld + 2) / 2);
+// factors for n roots x 3 axes, and the quartet's accumulators (over the
+// factors when the kernel accumulates in registers, nl_regacc > 0 lanes per
+// quartet, and they fit there)
+inline size_t generic_lds_bytes(int la, int lb, int lc, int ld, int nl_regacc = 0) {
+    const size_t nabcd = (size_t)ncart(la) * ncart(lb) * ncart(lc) * ncart(ld);
     const size_t n = (la + lb + lc + ld) / 2 + 1;
     const size_t ni = (size_t)(la + lb + 1) * (lc + ld + 1), ng = (size_t)(la + 1) * (lb + 1) * (lc + 1) * (ld + 1);
-    return (n * 3 * (ni + ng) + nabcd) * sizeof(double);
+    const size_t fac = n * 3 * (ni + ng);
+    const bool alias = nl_regacc > 0 && gen_racc(la, lb, lc, ld, nl_regacc) && nabcd <= fac;
+    return (alias ? fac : fac + nabcd) * sizeof(double);
 }
 
 }  // namespace pk

@@ -367,7 +367,7 @@ int launch_generic(const pk::GenArgs &a, hipStream_t s) {
         const void *k = a.B2 ? g.k2 : g.k1;
         if (k) {
             static std::atomic<int> attr_done[2 * kQuartClasses];  // zero-initialised
-            const size_t bytes = pk::generic_lds_bytes(a.la, a.lb, a.lc, a.ld);
+            const size_t bytes = pk::generic_lds_bytes(a.la, a.lb, a.lc, a.ld, 64 * g.waves);
             std::atomic<int> &done = attr_done[2 * quart_class(cb, ck) + (a.B2 ? 1 : 0)];
             if (!done.load()) {
                 PK_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
