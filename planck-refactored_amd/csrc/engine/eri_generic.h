@@ -99,6 +99,13 @@ __device__ __forceinline__ void seg_reduce(int lane, int nout, int nterm, Term t
 // immediate LDS offset, so a term costs two LDS reads and one FMA (the class
 // of the kernel fixed at compile time).  emit(o, value) takes the sum.  NL
 // lanes share the outputs.
+// A scheduling barrier after every PK_GEN_CT_GROUP terms keeps the compiler
+// from hoisting all NTERM LDS reads ahead of their FMAs (measured: the two-wave
+// (32|31) kernel 256 VGPRs + 94 AGPRs -> 237 VGPRs, 5.2 -> 2.7 ms; (33|21)
+// 3.2 -> 1.8 ms on n-C10H22 / cc-pVTZ).
+#ifndef PK_GEN_CT_GROUP
+#define PK_GEN_CT_GROUP 16
+#endif
 template <int NOUT, int NTERM, int NL, typename AO, typename BO, typename Emit>
 __device__ __forceinline__ void contract_ct(int lane, const double *acc, const double *S, AO a_of, BO b_of, Emit emit) {
     for (int o = lane; o < NOUT; o += NL) {
@@ -107,6 +114,9 @@ __device__ __forceinline__ void contract_ct(int lane, const double *acc, const d
         sfor<0, NTERM>([&](auto t_) {
             constexpr int t = PK_CI(t_);
             s[t & 3] = fma(base[b_of(t_)], S[t], s[t & 3]);
+#if PK_GEN_CT_GROUP > 0
+            if constexpr ((t + 1) % PK_GEN_CT_GROUP == 0) __builtin_amdgcn_sched_barrier(0);
+#endif
         });
         emit(o, (s[0] + s[1]) + (s[2] + s[3]));
     }
@@ -187,8 +197,12 @@ constexpr int gen_min_waves(int la, int lb, int lc, int ld, int nl) {
 #ifndef PK_GEN_NW2_MIN
 #define PK_GEN_NW2_MIN 1800
 #endif
+#ifndef PK_GEN_NW4_MIN
+#define PK_GEN_NW4_MIN 1000000
+#endif
 constexpr int gen_waves(int la, int lb, int lc, int ld) {
-    return ncart(la) * ncart(lb) * ncart(lc) * ncart(ld) >= PK_GEN_NW2_MIN ? 2 : 1;
+    const int n = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld);
+    return n >= PK_GEN_NW4_MIN ? 4 : (n >= PK_GEN_NW2_MIN ? 2 : 1);
 }
 
 // NW waves per workgroup (NL = 64 NW lanes) share each quartet; NW > 1 needs
