@@ -64,8 +64,9 @@ def print_classes(cls):
 
 def bench_water_clusters(args):
     """One step = the J/K build of all 256 clusters: a batch context (merged
-    per-class quartet lists), autotuned launch shapes, hipGraph replay.  Beside
-    it: the same 256 builds one context at a time."""
+    per-class quartet lists), the engine's default launch shapes (autotuned with
+    --autotune), hipGraph replay.  Beside it: the same 256 builds one context at
+    a time."""
     import torch
     from pkfock.engine import FockEngine, PREC_MIXED
     from pkfock.host import Molecule
@@ -89,7 +90,8 @@ def bench_water_clusters(args):
     stream = torch.cuda.current_stream(dev)
     eng.set_stream(stream.cuda_stream)
     eng.jk_device(P.data_ptr(), J.data_ptr(), K.data_ptr())  # work lists
-    eng.autotune(reps=2)
+    if args.autotune:
+        eng.autotune(reps=2)
     eng.set_graph(True)
     setup_s = time.perf_counter() - t
     for _ in range(max(args.warmup, 1)):

@@ -72,6 +72,11 @@ constexpr int kSegWaves = 16;
 // runs in segment mode (6-31G* water clusters: 11-13; cc-pVTZ alkanes: 40-80)
 constexpr double kSegAltPrim = 24.0;
 constexpr size_t kSegLdsMax = 64 * 1024;
+// lanes per workgroup of a class list until pk_ctx_autotune picks its own;
+// one wave per workgroup for the segment-mode lists of a batch with at most
+// kSegOneWaveLds bytes of exchange rows (see launch_list)
+constexpr int kDefaultBlock = 256;
+constexpr size_t kSegOneWaveLds = 24 * 1024;
 constexpr int kPairClasses = (kMaxL + 1) * (kMaxL + 2) / 2;
 constexpr int kQuartClasses = kPairClasses * (kPairClasses + 1) / 2;
 
@@ -199,7 +204,8 @@ struct TuneEntry {
 struct ClassWork {
     int cls = 0;
     int prec = PK_PREC_FP64;  // PK_PREC_MIXED: the FP32 list of the class
-    int block = 256;          // lanes per workgroup (autotuned)
+    int block = 0;            // lanes per workgroup (autotuned or PK_BLOCK; 0: the default of the launch)
+    int block_run = 0;        // lanes per workgroup of the last launch
     int remap = 0;            // XCD-contiguous wave ranges (autotuned)
     int alt = 0;              // alternative kernel (autotuned): FP64 integrals for an FP32 list,
                               // the cooperative kernel for a per-lane Rys class
@@ -293,6 +299,8 @@ struct pk_ctx {
     bool side_ok = false;
     bool side_off = false;  // PK_SIDE_QUEUES=0
     double seg_alt_prim = kSegAltPrim;  // PK_SEG_ALT_PRIM
+    int block0 = kDefaultBlock;         // PK_BLOCK: lanes per workgroup of every list before autotuning
+    bool block_forced = false;          // PK_BLOCK given: no per-list default
     bool cost_learned = false;  // the lists' times have been measured (first large build)
     bool learning = false;      // the current build records per-list events for that
     bool profile = false;
@@ -511,8 +519,21 @@ int build_work(pk_ctx *c) {
     c->st.primitive_quartets = 0;
     c->st.model_flops = 0.0;
     const bool mixed = c->precision == PK_PREC_MIXED;
-    struct BraOut {
-        std::vector<int> kets;
+    // ord[m][class]: positions in by_cls[m][class] in ket order (okey
+    // ascending), so that a bra's kets -- a prefix range of the q order -- come
+    // out in ket order by one pass over it instead of a sort per bra
+    std::vector<std::vector<std::vector<int>>> ord(c->nmol, std::vector<std::vector<int>>(kPairClasses));
+    parallel_for((size_t)c->nmol, [&](size_t m) {
+        for (int cl = 0; cl < kPairClasses; ++cl) {
+            const std::vector<int> &v = by_cls[m][cl];
+            std::vector<int> &o = ord[m][cl];
+            o.resize(v.size());
+            for (size_t i = 0; i < v.size(); ++i) o[i] = (int)i;
+            std::sort(o.begin(), o.end(), [&](int x, int y) { return okey[v[x]] < okey[v[y]]; });
+        }
+    });
+    struct BraRange {
+        size_t k0 = 0, k1 = 0;  // positions [k0, k1) of the q-ordered kets
         int64_t nprimq = 0;
         double flops = 0.0;
     };
@@ -532,12 +553,12 @@ int build_work(pk_ctx *c) {
             if (tasks.empty()) continue;
             for (int prec : {PK_PREC_FP64, PK_PREC_MIXED}) {
                 if (prec == PK_PREC_MIXED && !split) continue;
-                std::vector<BraOut> outs(tasks.size());
+                // pass 1: each bra's ket range (binary searches only)
+                std::vector<BraRange> outs(tasks.size());
                 parallel_for(tasks.size(), [&](size_t t) {
                     const int m = tasks[t].first, ib = tasks[t].second;
                     const std::vector<int> &B = by_cls[m][cb], &Kl = by_cls[m][ck];
-                    const int b = B[ib];
-                    const double qb = c->pair_q[b];
+                    const double qb = c->pair_q[B[ib]];
                     // number of kets (q descending) with qb * qk >= thr
                     auto count = [&](double thr) {
                         size_t lo = 0, hi = Kl.size();
@@ -552,35 +573,21 @@ int build_work(pk_ctx *c) {
                     };
                     const size_t n_tol = count(c->tol);
                     const size_t n_hi = split ? std::min(count(c->cutoff), n_tol) : n_tol;
-                    const size_t k0 = prec == PK_PREC_FP64 ? 0 : n_hi;
-                    const size_t k1 = prec == PK_PREC_FP64 ? n_hi : n_tol;
-                    if (k1 <= k0) return;
-                    std::vector<std::pair<uint64_t, int>> kk(k1 - k0);
-                    for (size_t i = k0; i < k1; ++i) kk[i - k0] = {okey[Kl[i]], Kl[i]};
-                    std::sort(kk.begin(), kk.end());
-                    BraOut &o = outs[t];
-                    o.kets.resize(kk.size());
-                    const int kb = c->pairs[b].nprim;
-                    for (size_t i = 0; i < kk.size(); ++i) {
-                        o.kets[i] = kk[i].second;
-                        const double K = (double)kb * c->pairs[kk[i].second].nprim;
-                        o.nprimq += (int64_t)K;
-                        o.flops += pk::model_flops(f.la, f.lb, f.lc, f.ld, K);
-                    }
+                    BraRange &o = outs[t];
+                    o.k0 = prec == PK_PREC_FP64 ? 0 : n_hi;
+                    o.k1 = std::max(o.k0, prec == PK_PREC_FP64 ? n_hi : n_tol);
                 });
                 ClassWork w;
                 w.cls = cls;
                 w.prec = prec;
+                w.block = c->block_forced ? c->block0 : 0;
                 // offsets of every bra's kets and waves (prefix sums), then the
                 // flat arrays filled in parallel
                 std::vector<size_t> koff(tasks.size() + 1, 0), woff(tasks.size() + 1, 0);
                 for (size_t t = 0; t < tasks.size(); ++t) {
-                    const size_t n = outs[t].kets.size();
+                    const size_t n = outs[t].k1 - outs[t].k0;
                     koff[t + 1] = koff[t] + n;
                     woff[t + 1] = woff[t] + (n + group - 1) / group;
-                    w.nquart += (int64_t)n;
-                    w.nprimq += outs[t].nprimq;
-                    w.flops += outs[t].flops;
                 }
                 if (koff.back() > (size_t)INT32_MAX) return fail(PK_ERR_ARG, "build_work: more than 2^31 quartets in one class");
                 const size_t nw = woff.back();
@@ -588,21 +595,49 @@ int build_work(pk_ctx *c) {
                 // (it holds one index per quartet, up to GBs for large batches)
                 std::unique_ptr<int[]> kets(new int[std::max<size_t>(1, koff.back())]);
                 std::vector<int> packed(4 * nw);
+                // pass 2: every bra's kets in ket order (ordering group
+                // descending, then first and second function) straight into the
+                // flat array, and its waves
                 parallel_for(tasks.size(), [&](size_t t) {
-                    BraOut &o = outs[t];
-                    const size_t n = o.kets.size();
+                    BraRange &o = outs[t];
+                    const size_t n = o.k1 - o.k0;
                     if (n == 0) return;
-                    std::copy(o.kets.begin(), o.kets.end(), kets.get() + koff[t]);
-                    const int b = by_cls[tasks[t].first][cb][tasks[t].second];
+                    const int m = tasks[t].first;
+                    const std::vector<int> &Kl = by_cls[m][ck];
+                    int *dst = kets.get() + koff[t];
+                    if (Kl.size() <= 16 * n + 64) {
+                        size_t j = 0;
+                        for (const int i : ord[m][ck])
+                            if ((size_t)i >= o.k0 && (size_t)i < o.k1) {
+                                dst[j++] = Kl[i];
+                                if (j == n) break;
+                            }
+                    } else {
+                        std::vector<std::pair<uint64_t, int>> kk(n);
+                        for (size_t i = o.k0; i < o.k1; ++i) kk[i - o.k0] = {okey[Kl[i]], Kl[i]};
+                        std::sort(kk.begin(), kk.end());
+                        for (size_t i = 0; i < n; ++i) dst[i] = kk[i].second;
+                    }
+                    const int b = by_cls[m][cb][tasks[t].second];
+                    const int kb = c->pairs[b].nprim;
+                    for (size_t i = 0; i < n; ++i) {
+                        const double K = (double)kb * c->pairs[dst[i]].nprim;
+                        o.nprimq += (int64_t)K;
+                        o.flops += pk::model_flops(f.la, f.lb, f.lc, f.ld, K);
+                    }
                     size_t wi = woff[t];
                     for (size_t s0 = 0; s0 < n; s0 += group, ++wi) {
                         packed[wi] = b;
                         packed[nw + wi] = (int)(koff[t] + s0);
                         packed[2 * nw + wi] = (int)std::min<size_t>(group, n - s0);
-                        packed[3 * nw + wi] = tasks[t].first;
+                        packed[3 * nw + wi] = m;
                     }
-                    std::vector<int>().swap(o.kets);
                 });
+                for (size_t t = 0; t < tasks.size(); ++t) {
+                    w.nquart += (int64_t)(outs[t].k1 - outs[t].k0);
+                    w.nprimq += outs[t].nprimq;
+                    w.flops += outs[t].flops;
+                }
                 if (w.nquart == 0) continue;
                 w.nwaves = (int)nw;
                 int rc;
@@ -783,41 +818,53 @@ int launch_list(pk_ctx *c, ClassWork &w, const BuildBufs &sb, bool events, hipSt
         const int before = nl;
         const ClassFns &f = kClasses[w.cls];
         const bool fp32 = w.prec == PK_PREC_MIXED && !w.alt;
-        const int wpb = w.block / 64;
         // segment mode when the exchange rows of the call fit the LDS budget
         // (two rows sets for the one-pass UHF OS kernels); the low-L OS classes
         // compiled both ways take it for lists of few primitive quartets per
         // shell quartet (atomic-bound), the plain kernel otherwise
         const bool seg_alt = f.seg_alt && use_os(c, f) && (double)w.nprimq <= c->seg_alt_prim * (double)w.nquart;
+        const int nk = uhf && use_os(c, f) ? 2 : 1;
         {
-            const int nk = uhf && use_os(c, f) ? 2 : 1;
             const size_t rows = (size_t)nk * (pk::ncart(f.la) + pk::ncart(f.lb)) * w.kcols +
                                 (size_t)pk::ncart(f.la) * pk::ncart(f.lb);
             a.segmode = c->seg_on && (f.seg || seg_alt) && w.nseg > 0 && rows * sizeof(double) <= kSegLdsMax;
         }
         const bool alt = seg_alt && a.segmode;  // the segment-mode variant of a plain OS class
+        // lanes per workgroup: the autotuned (or PK_BLOCK) shape, else one wave
+        // per workgroup for the segment-mode lists of a batch of molecules whose
+        // exchange rows leave room for at least 6 such workgroups per CU, and
+        // kDefaultBlock otherwise.  Measured per class list: on the 256 water
+        // clusters (rows of 2-19 KB) every list is faster with one wave (batched
+        // build 356 -> 256 ms); on n-C10H22 / cc-pVTZ (one molecule, segments of
+        // many kets) most lists are faster with four ((pp|ss) 7.4 -> 3.5 ms,
+        // (dd|dp) 11.4 -> 7.4 ms; a few lose up to 0.8 ms)
+        const bool one_wave =
+            a.segmode && c->nmol > 1 && pk::jk_lds_bytes(a, f.la, f.lb, nk) <= kSegOneWaveLds;
+        const int block = w.block > 0 ? w.block : (one_wave ? 64 : kDefaultBlock);
+        w.block_run = block;
+        const int wpb = block / 64;
         int grid = a.segmode ? w.nseg : (w.nwaves + wpb - 1) / wpb;
         if (w.remap) grid = (grid + 7) / 8 * 8;
         if (uhf && use_os(c, f)) {
-            (alt ? f.jk_os_uhf_seg : f.jk_os_uhf)(a, grid, w.block, s, nl);
+            (alt ? f.jk_os_uhf_seg : f.jk_os_uhf)(a, grid, block, s, nl);
         } else if (uhf && f.has_tmpl) {
             // per-lane Rys kernels once per spin density: J of each into A
             // (summing to J of Pa + Pb), exchange into B and B2
             pk::JKArgs x = a;
             x.P = sb.PKa;
-            f.jk_rys(x, grid, w.block, s, nl);
+            f.jk_rys(x, grid, block, s, nl);
             x.P = sb.PKb;
             x.B = sb.B2;
-            f.jk_rys(x, grid, w.block, s, nl);
+            f.jk_rys(x, grid, block, s, nl);
         } else if (uhf) {
             // handled by the cooperative kernel below
         } else if (use_os(c, f)) {
             if (alt)
-                (fp32 ? f.jk_os32_seg : f.jk_os_seg)(a, grid, w.block, s, nl);
+                (fp32 ? f.jk_os32_seg : f.jk_os_seg)(a, grid, block, s, nl);
             else
-                (fp32 ? f.jk_os32 : f.jk_os)(a, grid, w.block, s, nl);
+                (fp32 ? f.jk_os32 : f.jk_os)(a, grid, block, s, nl);
         } else if (f.has_tmpl && !(w.alt && w.prec == PK_PREC_FP64)) {
-            (fp32 ? f.jk_rys32 : f.jk_rys)(a, grid, w.block, s, nl);
+            (fp32 ? f.jk_rys32 : f.jk_rys)(a, grid, block, s, nl);
         }
         if ((uhf && !use_os(c, f) && !f.has_tmpl) ||
             (!uhf && !use_os(c, f) && !(f.has_tmpl && !(w.alt && w.prec == PK_PREC_FP64)))) {
@@ -1189,6 +1236,12 @@ int ctx_build(const pk_basis_desc *descs, int nmol, pk_ctx **out) {
     // diagnostic knob: PK_SEG_ALT_PRIM overrides the primitive quartets per
     // shell quartet below which the low-L OS lists run in segment mode
     if (const char *e = std::getenv("PK_SEG_ALT_PRIM")) c->seg_alt_prim = std::atof(e);
+    // diagnostic knob: PK_BLOCK = 64, 128 or 256 lanes per workgroup of every
+    // class list until pk_ctx_autotune chooses per list
+    if (const char *e = std::getenv("PK_BLOCK")) {
+        const int b = std::atoi(e);
+        if (b == 64 || b == 128 || b == 256) c->block0 = b, c->block_forced = true;
+    }
     c->st.shell_pairs = (int64_t)c->pairs.size();
     int rc = PK_OK;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
@@ -1402,7 +1455,7 @@ int pk_ctx_class_stats(const pk_ctx *c, int i, pk_class_stats *o) {
     o->ms_total = w.ms_total;
     o->builds = w.builds;
     o->precision = w.prec;
-    o->block = w.block;
+    o->block = w.block > 0 ? w.block : w.block_run;
     o->remap = w.remap;
     o->alt = w.alt;
     o->kind = list_kind(c, w);
