@@ -179,10 +179,13 @@ constexpr bool gen_racc(int la, int lb, int lc, int ld, int nl) {
 // Waves per SIMD the compiler must leave room for: two (256 registers) for the
 // one-wave kernels that accumulate more than GEN_RACC_MAX components per lane
 // in registers (without the bound they take 256 VGPRs and a few AGPRs, one
-// wave per SIMD); one elsewhere.
+// wave per SIMD; held to three they spill 88 VGPRs to scratch); one elsewhere.
+#ifndef PK_GEN_RACC1_WAVES
+#define PK_GEN_RACC1_WAVES 2
+#endif
 constexpr int gen_min_waves(int la, int lb, int lc, int ld, int nl) {
     return la >= 0 && nl == 64 && gen_racc(la, lb, lc, ld, nl) && gen_regs_per_lane(la, lb, lc, ld, nl) > GEN_RACC_MAX
-               ? 2
+               ? PK_GEN_RACC1_WAVES
                : 1;
 }
 
