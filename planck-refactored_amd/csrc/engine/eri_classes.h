@@ -75,12 +75,17 @@ namespace pk {
 // the roots and 2-D integrals.  Measured per class on n-C10H22 / cc-pVTZ,
 // budget 64 against 48: 0.56-0.88x the time for the chunked d/f classes (fewer
 // chunks), and (fp|dp) gains a per-lane kernel (9 chunks; 0.63x the
-// cooperative kernel's time).
+// cooperative kernel's time); budget 80 against 64: 0.83-0.94x for most chunked
+// classes ((fp|ps) 4.9 -> 3.7 ms, (fp|pp) 5.1 -> 4.2 ms, (fp|dp) 7.0 -> 6.2 ms).
 #ifndef PK_ACC_BUDGET
-#define PK_ACC_BUDGET 64
+#define PK_ACC_BUDGET 80
 #endif
 constexpr int kAccBudget = PK_ACC_BUDGET;
 constexpr int kMaxChunks = 12;  // larger classes run in the cooperative kernel
+// Which f classes get per-lane kernels is decided at this budget: at 80,
+// (fp|dd) and (fp|fp) would fit 12 and 10 chunks, but (fp|fp) is slower so
+// (2.0 -> 2.7 ms) and the two triple the kernel shards' build time.
+constexpr int kTmplBudget = 64;
 
 #ifndef PK_ACC_BUDGET32
 #define PK_ACC_BUDGET32 128
@@ -103,7 +108,7 @@ constexpr int chunk_size(int la, int lb, int lc, int ld, int budget = kAccBudget
 // classes with a per-lane Rys kernel: every s/p/d class, and f classes of at
 // most kMaxChunks ket-component chunks (the rest run in the cooperative kernel)
 constexpr bool rys_tmpl_available(int la, int lb, int lc, int ld) {
-    const int ncd = ncart(lc) * ncart(ld), ch = chunk_size(la, lb, lc, ld);
+    const int ncd = ncart(lc) * ncart(ld), ch = chunk_size(la, lb, lc, ld, kTmplBudget);
     return la <= 2 || (ncd + ch - 1) / ch <= kMaxChunks;
 }
 
